@@ -1,0 +1,813 @@
+// HIP kernels of libfem_amd for gfx950 (MI355X, CDNA4, wave64) and their C-ABI launchers.
+//
+// Element math (LINEAR plane stress, reference element/FEKernels.h:229-315, closed forms of the A2D
+// calls it makes):
+//   J = sum_a x_a g_a^T, g_a = dN_a/dxi;  detJ signed;  gradN_a = J^-T g_a
+//   grad u = sum_a u_a gradN_a^T;  eps = sym(grad u);  sigma = 2 mu eps + lambda tr(eps) I
+//   s_q = w_q detJ_q t
+//   R_{a,i}        = sum_q s_q sum_j sigma_ij gradN_{a,j}                         (FEKernels.h:106-197)
+//   K_{(a,i),(b,k)} = sum_q s_q [lambda gN_{a,i} gN_{b,k} + mu (delta_ik gN_a.gN_b + gN_{a,k} gN_{b,i})]
+// K is formed from the weighted Gram sums S_xx, S_yy, S_xy, S_yx of the physical gradients.
+//
+// Work mapping (all paths): a wavefront processes EPW = floor(64/N) elements at once; lane
+// (el, idx) = (lane / N, lane % N). idx is a node in the gather, a quadrature point in the geometry
+// stage and a column node b in the gradient and stiffness stages. Per-wave LDS holds the gathered
+// element nodes, per-quadrature-point geometry and the weighted gradients s_q*gradN_a(q).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "fem_internal.h"
+
+namespace fem {
+
+#define FEM_HIP_CHECK(expr)                                                                               \
+  do {                                                                                                    \
+    hipError_t err_ = (expr);                                                                             \
+    if (err_ != hipSuccess) {                                                                             \
+      return set_error(FEM_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(err_), __FILE__,    \
+                       __LINE__);                                                                         \
+    }                                                                                                     \
+  } while (0)
+
+template <int P>
+struct Ord {
+  static constexpr int NP = P + 1;
+  static constexpr int N = NP * NP;
+  static constexpr int Q = N;
+  static constexpr int EPW = 64 / N;
+  static constexpr int LANES = EPW * N;
+};
+
+// 1-D basis at the Gauss points: phi[k][i] = N_k(xi_i), dphi[k][i] = dN_k/dxi(xi_i), w[i].
+struct Basis1D {
+  double phi[5][5];
+  double dphi[5][5];
+  double w[5];
+};
+
+struct Material {
+  double mu, lambda, t;
+};
+
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+
+// Per-wave LDS staging.
+template <int P, typename T>
+struct __align__(16) WaveLDS {
+  using O = Ord<P>;
+  typename Vec2<T>::type sg[O::EPW][O::Q][O::N];  // s_q * gradN_a(q)
+  double nodal[O::EPW][O::N][4];                   // x, y, u, v of each element node
+  double geo[O::EPW][O::Q][8];                     // J^-1 (row-major), s_q, s_q*sigma_xx, s_q*sigma_yy, s_q*sigma_xy
+  int own[O::EPW][O::N];                           // owner path: local row of node a in this part, or -1
+};
+
+__device__ __forceinline__ void wave_sync() { __syncwarp(); }
+
+__device__ __forceinline__ void copy_basis(Basis1D &dst, const Basis1D &src) {
+  const double *s = reinterpret_cast<const double *>(&src);
+  double *d = reinterpret_cast<double *>(&dst);
+  constexpr int n = int(sizeof(Basis1D) / sizeof(double));
+  for (int i = threadIdx.x; i < n; i += blockDim.x) d[i] = s[i];
+}
+
+// Geometry and stress at quadrature point q of element `el` (lane (el, q)).
+template <int P, typename T>
+__device__ __forceinline__ void stage_geometry(WaveLDS<P, T> &W, const Basis1D &B, const Material &m, int el, int q) {
+  constexpr int NP = Ord<P>::NP;
+  const int qx = q % NP, qy = q / NP;
+  double px[NP], dpx[NP], py[NP], dpy[NP];
+#pragma unroll
+  for (int k = 0; k < NP; k++) {
+    px[k] = B.phi[k][qx];
+    dpx[k] = B.dphi[k][qx];
+    py[k] = B.phi[k][qy];
+    dpy[k] = B.dphi[k][qy];
+  }
+  double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
+  T U00 = 0, U01 = 0, U10 = 0, U11 = 0;
+#pragma unroll
+  for (int ay = 0; ay < NP; ay++) {
+#pragma unroll
+    for (int ax = 0; ax < NP; ax++) {
+      const int a = ay * NP + ax;
+      const double gx = py[ay] * dpx[ax];
+      const double gy = px[ax] * dpy[ay];
+      const double2 xy = *reinterpret_cast<const double2 *>(&W.nodal[el][a][0]);
+      const double2 uv = *reinterpret_cast<const double2 *>(&W.nodal[el][a][2]);
+      J00 = fma(xy.x, gx, J00);
+      J01 = fma(xy.x, gy, J01);
+      J10 = fma(xy.y, gx, J10);
+      J11 = fma(xy.y, gy, J11);
+      const T tgx = T(gx), tgy = T(gy), tu = T(uv.x), tv = T(uv.y);
+      U00 = fma(tu, tgx, U00);
+      U01 = fma(tu, tgy, U01);
+      U10 = fma(tv, tgx, U10);
+      U11 = fma(tv, tgy, U11);
+    }
+  }
+  const double detJ = J00 * J11 - J01 * J10;
+  const double inv = 1.0 / detJ;
+  const double i00 = J11 * inv, i01 = -J01 * inv, i10 = -J10 * inv, i11 = J00 * inv;
+  const T t00 = T(i00), t01 = T(i01), t10 = T(i10), t11 = T(i11);
+  const T G00 = U00 * t00 + U01 * t10;
+  const T G01 = U00 * t01 + U01 * t11;
+  const T G10 = U10 * t00 + U11 * t10;
+  const T G11 = U10 * t01 + U11 * t11;
+  const T mu = T(m.mu), lam = T(m.lambda);
+  const T tr = G00 + G11;
+  const T sxx = 2 * mu * G00 + lam * tr;
+  const T syy = 2 * mu * G11 + lam * tr;
+  const T sxy = mu * (G01 + G10);
+  const double s = (B.w[qx] * B.w[qy]) * detJ * m.t;
+  const T ts = T(s);
+  double *g = W.geo[el][q];
+  *reinterpret_cast<double2 *>(g + 0) = make_double2(i00, i01);
+  *reinterpret_cast<double2 *>(g + 2) = make_double2(i10, i11);
+  *reinterpret_cast<double2 *>(g + 4) = make_double2(s, double(ts * sxx));
+  *reinterpret_cast<double2 *>(g + 6) = make_double2(double(ts * syy), double(ts * sxy));
+}
+
+// Physical gradients of basis function b at every quadrature point (lane (el, b)); residual entries
+// of node b; writes s_q * gradN_b(q) to LDS for the stiffness stage.
+template <int P, typename T>
+__device__ __forceinline__ void stage_gradients(WaveLDS<P, T> &W, const Basis1D &B, int el, int b,
+                                                T (&gN)[Ord<P>::Q][2], T &Rx, T &Ry) {
+  constexpr int NP = Ord<P>::NP;
+  const int bx = b % NP, by = b / NP;
+  T pbx[NP], dpbx[NP], pby[NP], dpby[NP];
+#pragma unroll
+  for (int i = 0; i < NP; i++) {
+    pbx[i] = T(B.phi[bx][i]);
+    dpbx[i] = T(B.dphi[bx][i]);
+    pby[i] = T(B.phi[by][i]);
+    dpby[i] = T(B.dphi[by][i]);
+  }
+  Rx = 0;
+  Ry = 0;
+#pragma unroll
+  for (int qy = 0; qy < NP; qy++) {
+#pragma unroll
+    for (int qx = 0; qx < NP; qx++) {
+      const int q = qy * NP + qx;
+      const T gx = pby[qy] * dpbx[qx];
+      const T gy = pbx[qx] * dpby[qy];
+      const double *g = W.geo[el][q];
+      const double2 ia = *reinterpret_cast<const double2 *>(g + 0);
+      const double2 ib = *reinterpret_cast<const double2 *>(g + 2);
+      const double2 sc = *reinterpret_cast<const double2 *>(g + 4);
+      const double2 sd = *reinterpret_cast<const double2 *>(g + 6);
+      const T nx = gx * T(ia.x) + gy * T(ib.x);
+      const T ny = gx * T(ia.y) + gy * T(ib.y);
+      gN[q][0] = nx;
+      gN[q][1] = ny;
+      const T txx = T(sc.y), tyy = T(sd.x), txy = T(sd.y);
+      Rx += txx * nx + txy * ny;
+      Ry += txy * nx + tyy * ny;
+      const T s = T(sc.x);
+      typename Vec2<T>::type v;
+      v.x = s * nx;
+      v.y = s * ny;
+      W.sg[el][q][b] = v;
+    }
+  }
+}
+
+// 2x2 stiffness block K(a, b) for lane (el, b), row-major [xx, xy, yx, yy].
+template <int P, typename T>
+__device__ __forceinline__ void stage_block(const WaveLDS<P, T> &W, const Material &m, int el, int a,
+                                            const T (&gN)[Ord<P>::Q][2], double (&k)[4]) {
+  constexpr int Q = Ord<P>::Q;
+  T Sxx = 0, Syy = 0, Sxy = 0, Syx = 0;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const typename Vec2<T>::type v = W.sg[el][q][a];
+    Sxx = fma(v.x, gN[q][0], Sxx);
+    Syy = fma(v.y, gN[q][1], Syy);
+    Sxy = fma(v.x, gN[q][1], Sxy);
+    Syx = fma(v.y, gN[q][0], Syx);
+  }
+  const T mu = T(m.mu), lam = T(m.lambda), c1 = lam + 2 * mu;
+  k[0] = double(c1 * Sxx + mu * Syy);
+  k[1] = double(lam * Sxy + mu * Syx);
+  k[2] = double(lam * Syx + mu * Sxy);
+  k[3] = double(c1 * Syy + mu * Sxx);
+}
+
+// ---- element-parallel kernel (drop-in atomic path and colour path) -----------------------------
+struct ElemArgs {
+  const int *connPtr;
+  const int *conn;
+  const int *elemList;  // colour path: element ids of this colour; NULL: elements first..first+count-1
+  int first;
+  int count;
+  const double *states;
+  const double *coords;
+  const int *map;
+  double *residual;
+  double *values;
+  Basis1D basis;
+  Material mat;
+};
+
+template <int P, typename T, int WAVES, bool RES, bool JAC, bool ATOMIC>
+__global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args) {
+  using O = Ord<P>;
+  constexpr int N = O::N;
+  __shared__ WaveLDS<P, T> wl[WAVES];
+  __shared__ Basis1D basis;
+  copy_basis(basis, args.basis);
+  __syncthreads();
+
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int el = lane / N, idx = lane % N;
+  WaveLDS<P, T> &W = wl[wave];
+  const int64_t slot = (int64_t(blockIdx.x) * WAVES + wave) * O::EPW + el;
+  const bool valid = lane < O::LANES && slot < args.count;
+  int eid = 0, node = 0;
+  if (valid) {
+    eid = args.elemList ? args.elemList[slot] : args.first + int(slot);
+    node = args.conn[args.connPtr[eid] + idx];
+    const double2 xy = *reinterpret_cast<const double2 *>(args.coords + 2 * int64_t(node));
+    const double2 uv = *reinterpret_cast<const double2 *>(args.states + 2 * int64_t(node));
+    *reinterpret_cast<double2 *>(&W.nodal[el][idx][0]) = xy;
+    *reinterpret_cast<double2 *>(&W.nodal[el][idx][2]) = uv;
+  }
+  wave_sync();
+  if (valid) stage_geometry<P, T>(W, basis, args.mat, el, idx);
+  wave_sync();
+  T gN[O::Q][2];
+  T Rx = 0, Ry = 0;
+  if (valid) stage_gradients<P, T>(W, basis, el, idx, gN, Rx, Ry);
+  if (RES && valid) {
+    double *r = args.residual + 2 * int64_t(node);
+    if (ATOMIC) {
+      unsafeAtomicAdd(r, double(Rx));
+      unsafeAtomicAdd(r + 1, double(Ry));
+    } else {
+      r[0] += double(Rx);
+      r[1] += double(Ry);
+    }
+  }
+  if (JAC) {
+    wave_sync();
+    if (valid) {
+      const int *mrow = args.map + int64_t(eid) * N * N + idx;
+      for (int a = 0; a < N; a++) {
+        double k[4];
+        stage_block<P, T>(W, args.mat, el, a, gN, k);
+        double *blk = args.values + mrow[a * N];
+        if (ATOMIC) {
+#pragma unroll
+          for (int c = 0; c < 4; c++) unsafeAtomicAdd(blk + c, k[c]);
+        } else {
+#pragma unroll
+          for (int c = 0; c < 4; c++) blk[c] += k[c];
+        }
+      }
+    }
+  }
+}
+
+// ---- owner-computes kernel (resident plan, write-once) --------------------------------------------
+struct OwnerArgs {
+  const int *conn;  // dense [E*N]
+  const int *nodeLrow;
+  const int *partRowPtr;
+  const int *rows;
+  const int *rowLbase;
+  const int *partBlocks;
+  const int *partRunPtr;
+  const int4 *runs;
+  const int *partElemPtr;
+  const int *elems;
+  const int *colourOff;
+  const uint8_t *rank;
+  int numParts;
+  int numColours;
+  const double *states;
+  const double *coords;
+  double *residual;
+  double *values;
+  Basis1D basis;
+  Material mat;
+};
+
+template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC, bool DET>
+__device__ __forceinline__ void owner_batch(const OwnerArgs &args, WaveLDS<P, T> &W, const Basis1D &basis,
+                                            double *acc, double *racc, const int *lrows, const int *lbase, int nr,
+                                            int e0, int bstart, int bend, int el, int idx, int lane) {
+  using O = Ord<P>;
+  constexpr int N = O::N;
+  const bool valid = lane < O::LANES && bstart + el < bend;
+  int eid = 0, own = -1;
+  if (valid) {
+    eid = args.elems[e0 + bstart + el];
+    const int node = args.conn[int64_t(eid) * N + idx];
+    const int lr = args.nodeLrow[node];
+    own = (lr < nr && lrows[lr] == node) ? lr : -1;
+    const double2 xy = *reinterpret_cast<const double2 *>(args.coords + 2 * int64_t(node));
+    const double2 uv = *reinterpret_cast<const double2 *>(args.states + 2 * int64_t(node));
+    *reinterpret_cast<double2 *>(&W.nodal[el][idx][0]) = xy;
+    *reinterpret_cast<double2 *>(&W.nodal[el][idx][2]) = uv;
+    W.own[el][idx] = own;
+  }
+  wave_sync();
+  if (valid) stage_geometry<P, T>(W, basis, args.mat, el, idx);
+  wave_sync();
+  T gN[O::Q][2];
+  T Rx = 0, Ry = 0;
+  if (valid) stage_gradients<P, T>(W, basis, el, idx, gN, Rx, Ry);
+  if (RES && valid && own >= 0) {
+    if (DET) {
+      racc[2 * own] += double(Rx);
+      racc[2 * own + 1] += double(Ry);
+    } else {
+      atomicAdd(&racc[2 * own], double(Rx));
+      atomicAdd(&racc[2 * own + 1], double(Ry));
+    }
+  }
+  if (JAC) {
+    wave_sync();
+    if (valid) {
+      const uint8_t *rk = args.rank + int64_t(eid) * N * N + idx;
+      for (int a = 0; a < N; a++) {
+        const int ownA = W.own[el][a];
+        if (ownA < 0) continue;
+        double k[4];
+        stage_block<P, T>(W, args.mat, el, a, gN, k);
+        double *dst = acc + 4 * (lbase[ownA] + int(rk[a * N]));
+        if (DET) {
+#pragma unroll
+          for (int c = 0; c < 4; c++) dst[c] += k[c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 4; c++) atomicAdd(dst + c, k[c]);
+        }
+      }
+    }
+  }
+  wave_sync();
+}
+
+template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC, bool DET>
+__global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args) {
+  using O = Ord<P>;
+  __shared__ WaveLDS<P, T> wl[WAVES];
+  __shared__ Basis1D basis;
+  __shared__ __align__(16) double acc[JAC ? 4 * MAXB : 2];
+  __shared__ __align__(16) double racc[RES ? 2 * MAXR : 2];
+  __shared__ int lrows[MAXR];
+  __shared__ int lbase[MAXR];
+
+  // XCD-aware order: consecutive workgroup ids land on different XCDs, so give each XCD a
+  // contiguous range of (spatially adjacent) parts to keep shared element data in its L2.
+  const int nxcd = 8;
+  const int per = int(gridDim.x) / nxcd;
+  const int part = int(blockIdx.x % nxcd) * per + int(blockIdx.x / nxcd);
+  if (part >= args.numParts) return;
+
+  const int r0 = args.partRowPtr[part];
+  const int nr = args.partRowPtr[part + 1] - r0;
+  const int nb = args.partBlocks[part];
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) {
+    lrows[i] = args.rows[r0 + i];
+    lbase[i] = args.rowLbase[r0 + i];
+  }
+  if (JAC) {
+    for (int i = threadIdx.x; i < 2 * nb; i += blockDim.x) reinterpret_cast<double2 *>(acc)[i] = make_double2(0, 0);
+  }
+  if (RES) {
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
+  }
+  copy_basis(basis, args.basis);
+  __syncthreads();
+
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int el = lane / O::N, idx = lane % O::N;
+  WaveLDS<P, T> &W = wl[wave];
+  const int e0 = args.partElemPtr[part];
+  const int ne = args.partElemPtr[part + 1] - e0;
+  if (DET) {
+    const int *off = args.colourOff + int64_t(part) * (args.numColours + 1);
+    for (int c = 0; c < args.numColours; c++) {
+      const int cb = off[c], ce = off[c + 1];
+      for (int b = cb + wave * O::EPW; b < ce; b += WAVES * O::EPW) {
+        owner_batch<P, T, WAVES, MAXB, MAXR, RES, JAC, DET>(args, W, basis, acc, racc, lrows, lbase, nr, e0, b, ce, el,
+                                                            idx, lane);
+      }
+      __syncthreads();
+    }
+  } else {
+    for (int b = wave * O::EPW; b < ne; b += WAVES * O::EPW) {
+      owner_batch<P, T, WAVES, MAXB, MAXR, RES, JAC, DET>(args, W, basis, acc, racc, lrows, lbase, nr, e0, b, ne, el,
+                                                          idx, lane);
+    }
+    __syncthreads();
+  }
+
+  // Write-once output: BSR runs are contiguous in LDS and in the value array.
+  if (JAC) {
+    const int rb = args.partRunPtr[part], re = args.partRunPtr[part + 1];
+    for (int r = rb; r < re; r++) {
+      const int4 run = args.runs[r];
+      const double2 *src = reinterpret_cast<const double2 *>(acc) + 2 * run.x;
+      double2 *dst = reinterpret_cast<double2 *>(args.values) + 2 * int64_t(run.y);
+      for (int i = threadIdx.x; i < 2 * run.z; i += blockDim.x) dst[i] = src[i];
+    }
+  }
+  if (RES) {
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) {
+      reinterpret_cast<double2 *>(args.residual)[lrows[i]] = reinterpret_cast<const double2 *>(racc)[i];
+    }
+  }
+}
+
+// ---- BSR SpMV ---------------------------------------------------------------------------------------
+__global__ void bsr_spmv_kernel(int numRows, const int *rowp, const int *cols, const double *values, const double *x,
+                                double *y) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= numRows) return;
+  double y0 = 0, y1 = 0;
+  for (int k = rowp[r]; k < rowp[r + 1]; k++) {
+    const double *A = values + 4 * int64_t(k);
+    const int c = cols[k];
+    const double x0 = x[2 * int64_t(c)], x1 = x[2 * int64_t(c) + 1];
+    y0 += A[0] * x0 + A[1] * x1;
+    y1 += A[2] * x0 + A[3] * x1;
+  }
+  y[2 * int64_t(r)] = y0;
+  y[2 * int64_t(r) + 1] = y1;
+}
+
+// ---- host helpers -------------------------------------------------------------------------------------
+namespace {
+
+int make_basis(int order, Basis1D &b) {
+  std::memset(&b, 0, sizeof(b));
+  double phi[25], dphi[25], xg[5], wg[5];
+  int rc = fem_basis_table(order, phi, dphi, xg, wg);
+  if (rc) return rc;
+  const int np = order + 1;
+  for (int k = 0; k < np; k++) {
+    for (int i = 0; i < np; i++) {
+      b.phi[k][i] = phi[k * np + i];
+      b.dphi[k][i] = dphi[k * np + i];
+    }
+  }
+  for (int i = 0; i < np; i++) b.w[i] = wg[i];
+  return FEM_OK;
+}
+
+int make_material(double E, double nu, double t, Material &m) {
+  if (!(E > 0) || !(nu > -1.0 && nu < 0.5) || !(t > 0)) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "material E=%g nu=%g t=%g out of range", E, nu, t);
+  }
+  // Plane stress: mu = E / (2 (1 + nu)), lambda = 2 mu nu / (1 - nu)  (FEKernels.h:300-301)
+  m.mu = 0.5 * E / (1.0 + nu);
+  m.lambda = 2 * m.mu * nu / (1.0 - nu);
+  m.t = t;
+  return FEM_OK;
+}
+
+// Records events around `launch` when timing is requested.
+template <typename Fn>
+int timed(hipStream_t stream, double *elapsedSeconds, Fn launch) {
+  hipEvent_t start = nullptr, stop = nullptr;
+  if (elapsedSeconds) {
+    FEM_HIP_CHECK(hipEventCreate(&start));
+    FEM_HIP_CHECK(hipEventCreate(&stop));
+    FEM_HIP_CHECK(hipEventRecord(start, stream));
+  }
+  int rc = launch();
+  if (rc == FEM_OK) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = set_error(FEM_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+  }
+  if (elapsedSeconds) {
+    if (rc == FEM_OK) {
+      hipError_t e = hipEventRecord(stop, stream);
+      if (e == hipSuccess) e = hipEventSynchronize(stop);
+      float ms = 0;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, start, stop);
+      if (e != hipSuccess) rc = set_error(FEM_ERR_HIP, "timing events failed: %s", hipGetErrorString(e));
+      *elapsedSeconds = double(ms) * 1e-3;
+    }
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
+  }
+  return rc;
+}
+
+constexpr int elem_waves(int order) { return order <= 2 ? 4 : 2; }
+
+template <int P, typename T, bool RES, bool JAC, bool ATOMIC>
+void launch_element(const ElemArgs &a, hipStream_t stream) {
+  constexpr int W = elem_waves(P);
+  constexpr int per = W * Ord<P>::EPW;
+  const int grid = (a.count + per - 1) / per;
+  if (grid > 0) element_kernel<P, T, W, RES, JAC, ATOMIC><<<grid, W * 64, 0, stream>>>(a);
+}
+
+template <int P, bool RES, bool JAC, bool ATOMIC>
+void launch_element_prec(const ElemArgs &a, int precision, hipStream_t stream) {
+  if (precision == FEM_PREC_MIXED) {
+    launch_element<P, float, RES, JAC, ATOMIC>(a, stream);
+  } else {
+    launch_element<P, double, RES, JAC, ATOMIC>(a, stream);
+  }
+}
+
+template <bool RES, bool JAC, bool ATOMIC>
+void launch_element_order(int order, const ElemArgs &a, int precision, hipStream_t stream) {
+  switch (order) {
+    case 1: launch_element_prec<1, RES, JAC, ATOMIC>(a, precision, stream); break;
+    case 2: launch_element_prec<2, RES, JAC, ATOMIC>(a, precision, stream); break;
+    case 3: launch_element_prec<3, RES, JAC, ATOMIC>(a, precision, stream); break;
+    default: launch_element_prec<4, RES, JAC, ATOMIC>(a, precision, stream); break;
+  }
+}
+
+template <int P, typename T, bool RES, bool JAC, bool DET>
+void launch_owner(const OwnerArgs &a, hipStream_t stream) {
+  constexpr OwnerConfig cfg = owner_config(P);
+  constexpr int MAXR = (JAC || !RES) ? cfg.maxRows : kResidualOnlyMaxRows;
+  const int grid = ((a.numParts + 7) / 8) * 8;
+  owner_kernel<P, T, cfg.waves, cfg.maxBlocks, MAXR, RES, JAC, DET><<<grid, cfg.waves * 64, 0, stream>>>(a);
+}
+
+template <int P, typename T, bool DET>
+void launch_owner_mode(const OwnerArgs &a, bool res, bool jac, bool residualOnlyParts, hipStream_t stream) {
+  if (jac && res) {
+    launch_owner<P, T, true, true, DET>(a, stream);
+  } else if (jac) {
+    launch_owner<P, T, false, true, DET>(a, stream);
+  } else if (residualOnlyParts) {
+    launch_owner<P, T, true, false, DET>(a, stream);
+  } else {
+    // Residual from a plan whose parts were sized for the Jacobian: rows fit the smaller bound.
+    constexpr OwnerConfig cfg = owner_config(P);
+    const int grid = ((a.numParts + 7) / 8) * 8;
+    owner_kernel<P, T, cfg.waves, cfg.maxBlocks, cfg.maxRows, true, false, DET><<<grid, cfg.waves * 64, 0, stream>>>(a);
+  }
+}
+
+template <int P>
+void launch_owner_order(const OwnerArgs &a, int precision, bool det, bool res, bool jac, bool rOnly,
+                        hipStream_t stream) {
+  if (precision == FEM_PREC_MIXED) {
+    if (det) launch_owner_mode<P, float, true>(a, res, jac, rOnly, stream);
+    else launch_owner_mode<P, float, false>(a, res, jac, rOnly, stream);
+  } else {
+    if (det) launch_owner_mode<P, double, true>(a, res, jac, rOnly, stream);
+    else launch_owner_mode<P, double, false>(a, res, jac, rOnly, stream);
+  }
+}
+
+int check_common(int order, const int *connPtr, const int *conn, int numElements, const double *states,
+                 const double *nodeCoords, int precision) {
+  if (order < 1 || order > 4) return set_error(FEM_ERR_INVALID_ORDER, "element order %d outside 1..4", order);
+  if (numElements < 0) return set_error(FEM_ERR_INVALID_ARGUMENT, "numElements %d < 0", numElements);
+  if (precision != FEM_PREC_FP64 && precision != FEM_PREC_MIXED) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "unknown precision mode %d", precision);
+  }
+  if (numElements > 0 && (!connPtr || !conn || !states || !nodeCoords)) {
+    return set_error(FEM_ERR_NULL_POINTER, "connPtr/conn/states/nodeCoords is NULL");
+  }
+  return FEM_OK;
+}
+
+template <typename TPtr>
+int upload(const std::vector<TPtr> &h, TPtr **d, int64_t &bytes) {
+  const size_t n = std::max<size_t>(h.size(), 1) * sizeof(TPtr);
+  FEM_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(d), n));
+  if (!h.empty()) FEM_HIP_CHECK(hipMemcpy(*d, h.data(), h.size() * sizeof(TPtr), hipMemcpyHostToDevice));
+  bytes += int64_t(n);
+  return FEM_OK;
+}
+
+void free_plan_arrays(fem_plan *p) {
+  void *ptrs[] = {p->dConn,    p->dNodeLrow,    p->dPartRowPtr, p->dRows,  p->dRowLbase, p->dPartBlocks,
+                  p->dPartRunPtr, p->dRuns,     p->dPartElemPtr, p->dElems, p->dColourOff, p->dRank};
+  for (void *q : ptrs) {
+    if (q) (void)hipFree(q);
+  }
+}
+
+}  // namespace
+}  // namespace fem
+
+using namespace fem;
+
+extern "C" {
+
+int fem_residual(int order, const int *connPtr, const int *conn, int numElements, const double *states,
+                 const double *nodeCoords, double E, double nu, double t, double *residual, int precision,
+                 void *stream, double *elapsedSeconds) {
+  int rc = check_common(order, connPtr, conn, numElements, states, nodeCoords, precision);
+  if (rc) return rc;
+  if (numElements > 0 && !residual) return set_error(FEM_ERR_NULL_POINTER, "residual is NULL");
+  ElemArgs a{};
+  a.connPtr = connPtr;
+  a.conn = conn;
+  a.elemList = nullptr;
+  a.first = 0;
+  a.count = numElements;
+  a.states = states;
+  a.coords = nodeCoords;
+  a.residual = residual;
+  if ((rc = make_basis(order, a.basis)) || (rc = make_material(E, nu, t, a.mat))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return timed(s, elapsedSeconds, [&]() {
+    launch_element_order<true, false, true>(order, a, precision, s);
+    return FEM_OK;
+  });
+}
+
+int fem_jacobian(int order, const int *connPtr, const int *conn, int numElements, const double *states,
+                 const double *nodeCoords, double E, double nu, double t, const int *elementBCSRMap,
+                 double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds) {
+  int rc = check_common(order, connPtr, conn, numElements, states, nodeCoords, precision);
+  if (rc) return rc;
+  if (numElements > 0 && (!elementBCSRMap || !matEntries)) {
+    return set_error(FEM_ERR_NULL_POINTER, "elementBCSRMap/matEntries is NULL");
+  }
+  ElemArgs a{};
+  a.connPtr = connPtr;
+  a.conn = conn;
+  a.elemList = nullptr;
+  a.first = 0;
+  a.count = numElements;
+  a.states = states;
+  a.coords = nodeCoords;
+  a.map = elementBCSRMap;
+  a.residual = residual;
+  a.values = matEntries;
+  if ((rc = make_basis(order, a.basis)) || (rc = make_material(E, nu, t, a.mat))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return timed(s, elapsedSeconds, [&]() {
+    if (residual) launch_element_order<true, true, true>(order, a, precision, s);
+    else launch_element_order<false, true, true>(order, a, precision, s);
+    return FEM_OK;
+  });
+}
+
+int fem_jacobian_coloured(int order, const int *connPtr, const int *conn, int numElements, const double *states,
+                          const double *nodeCoords, double E, double nu, double t, const int *elementBCSRMap,
+                          const int *colourElements, const int *colourPtr, int numColours, double *residual,
+                          double *matEntries, int precision, void *stream, double *elapsedSeconds) {
+  int rc = check_common(order, connPtr, conn, numElements, states, nodeCoords, precision);
+  if (rc) return rc;
+  if (numElements > 0 && (!elementBCSRMap || !matEntries || !colourElements || !colourPtr)) {
+    return set_error(FEM_ERR_NULL_POINTER, "elementBCSRMap/matEntries/colourElements/colourPtr is NULL");
+  }
+  if (numColours < 0 || (numElements > 0 && numColours == 0)) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "numColours %d invalid", numColours);
+  }
+  for (int c = 0; c < numColours; c++) {
+    if (colourPtr[c + 1] < colourPtr[c]) return set_error(FEM_ERR_INVALID_ARGUMENT, "colourPtr not ascending");
+  }
+  if (numColours > 0 && (colourPtr[0] != 0 || colourPtr[numColours] != numElements)) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "colourPtr must span 0..numElements");
+  }
+  ElemArgs a{};
+  a.connPtr = connPtr;
+  a.conn = conn;
+  a.states = states;
+  a.coords = nodeCoords;
+  a.map = elementBCSRMap;
+  a.residual = residual;
+  a.values = matEntries;
+  if ((rc = make_basis(order, a.basis)) || (rc = make_material(E, nu, t, a.mat))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return timed(s, elapsedSeconds, [&]() {
+    for (int c = 0; c < numColours; c++) {
+      a.elemList = colourElements + colourPtr[c];
+      a.first = 0;
+      a.count = colourPtr[c + 1] - colourPtr[c];
+      if (residual) launch_element_order<true, true, false>(order, a, precision, s);
+      else launch_element_order<false, true, false>(order, a, precision, s);
+    }
+    return FEM_OK;
+  });
+}
+
+int fem_plan_create(int order, int numNodes, int numElements, const int *connPtr, const int *conn, const int *rowp,
+                    const int *cols, const double *nodeCoordsHost, int flags, fem_plan **plan) {
+  if (!plan) return set_error(FEM_ERR_NULL_POINTER, "plan is NULL");
+  *plan = nullptr;
+  PlanHost h;
+  int rc = build_plan_host(order, numNodes, numElements, connPtr, conn, rowp, cols, nodeCoordsHost, flags, h);
+  if (rc) return rc;
+  fem_plan *p = new fem_plan();
+  p->order = order;
+  p->numNodes = numNodes;
+  p->numElements = numElements;
+  p->numParts = h.numParts;
+  p->numColours = h.numColours;
+  p->flags = flags;
+  p->numBlocks = h.numBlocks;
+  p->elementVisits = int64_t(h.elems.size());
+  (void)hipGetDevice(&p->device);
+  int64_t bytes = 0;
+  if ((rc = upload(h.conn, &p->dConn, bytes)) || (rc = upload(h.nodeLrow, &p->dNodeLrow, bytes)) ||
+      (rc = upload(h.partRowPtr, &p->dPartRowPtr, bytes)) || (rc = upload(h.rows, &p->dRows, bytes)) ||
+      (rc = upload(h.rowLbase, &p->dRowLbase, bytes)) || (rc = upload(h.partBlocks, &p->dPartBlocks, bytes)) ||
+      (rc = upload(h.partRunPtr, &p->dPartRunPtr, bytes)) || (rc = upload(h.runs, &p->dRuns, bytes)) ||
+      (rc = upload(h.partElemPtr, &p->dPartElemPtr, bytes)) || (rc = upload(h.elems, &p->dElems, bytes)) ||
+      (rc = upload(h.colourOff, &p->dColourOff, bytes)) || (rc = upload(h.rank, &p->dRank, bytes))) {
+    free_plan_arrays(p);
+    delete p;
+    return rc;
+  }
+  p->deviceBytes = bytes;
+  *plan = p;
+  return FEM_OK;
+}
+
+int fem_plan_info(const fem_plan *plan, int64_t *numParts, int64_t *numBlocks, int64_t *elementVisits,
+                  int64_t *deviceBytes) {
+  if (!plan) return set_error(FEM_ERR_NULL_POINTER, "plan is NULL");
+  if (numParts) *numParts = plan->numParts;
+  if (numBlocks) *numBlocks = plan->numBlocks;
+  if (elementVisits) *elementVisits = plan->elementVisits;
+  if (deviceBytes) *deviceBytes = plan->deviceBytes;
+  return FEM_OK;
+}
+
+int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCoords, double E, double nu, double t,
+                      double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds) {
+  if (!plan) return set_error(FEM_ERR_NULL_POINTER, "plan is NULL");
+  if (!states || !nodeCoords) return set_error(FEM_ERR_NULL_POINTER, "states/nodeCoords is NULL");
+  if (!residual && !matEntries) return set_error(FEM_ERR_NULL_POINTER, "residual and matEntries are both NULL");
+  if (precision != FEM_PREC_FP64 && precision != FEM_PREC_MIXED) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "unknown precision mode %d", precision);
+  }
+  const bool rOnly = (plan->flags & FEM_PLAN_RESIDUAL_ONLY) != 0;
+  if (rOnly && matEntries) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "plan was built residual-only; it cannot assemble the Jacobian");
+  }
+  OwnerArgs a{};
+  a.conn = plan->dConn;
+  a.nodeLrow = plan->dNodeLrow;
+  a.partRowPtr = plan->dPartRowPtr;
+  a.rows = plan->dRows;
+  a.rowLbase = plan->dRowLbase;
+  a.partBlocks = plan->dPartBlocks;
+  a.partRunPtr = plan->dPartRunPtr;
+  a.runs = reinterpret_cast<const int4 *>(plan->dRuns);
+  a.partElemPtr = plan->dPartElemPtr;
+  a.elems = plan->dElems;
+  a.colourOff = plan->dColourOff;
+  a.rank = plan->dRank;
+  a.numParts = plan->numParts;
+  a.numColours = plan->numColours;
+  a.states = states;
+  a.coords = nodeCoords;
+  a.residual = residual;
+  a.values = matEntries;
+  int rc;
+  if ((rc = make_basis(plan->order, a.basis)) || (rc = make_material(E, nu, t, a.mat))) return rc;
+  const bool det = (plan->flags & FEM_PLAN_DETERMINISTIC) != 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return timed(s, elapsedSeconds, [&]() {
+    switch (plan->order) {
+      case 1: launch_owner_order<1>(a, precision, det, residual != nullptr, matEntries != nullptr, rOnly, s); break;
+      case 2: launch_owner_order<2>(a, precision, det, residual != nullptr, matEntries != nullptr, rOnly, s); break;
+      case 3: launch_owner_order<3>(a, precision, det, residual != nullptr, matEntries != nullptr, rOnly, s); break;
+      default: launch_owner_order<4>(a, precision, det, residual != nullptr, matEntries != nullptr, rOnly, s); break;
+    }
+    return FEM_OK;
+  });
+}
+
+int fem_plan_destroy(fem_plan *plan) {
+  if (!plan) return FEM_OK;
+  free_plan_arrays(plan);
+  delete plan;
+  return FEM_OK;
+}
+
+int fem_bsr_spmv(int numRows, const int *rowp, const int *cols, const double *values, const double *x, double *y,
+                 void *stream) {
+  if (numRows < 0) return set_error(FEM_ERR_INVALID_ARGUMENT, "numRows < 0");
+  if (numRows == 0) return FEM_OK;
+  if (!rowp || !cols || !values || !x || !y) return set_error(FEM_ERR_NULL_POINTER, "NULL pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  bsr_spmv_kernel<<<(numRows + 255) / 256, 256, 0, s>>>(numRows, rowp, cols, values, x, y);
+  FEM_HIP_CHECK(hipGetLastError());
+  return FEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// Internal declarations shared by the host code (fem_host.cpp) and the HIP code (fem_device.hip).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "fem_amd.h"
+
+namespace fem {
+
+// Records `code` and a printf-style message as the calling thread's last error; returns `code`.
+int set_error(int code, const char *fmt, ...);
+void clear_error();
+
+inline int nodes_per_element(int order) { return (order + 1) * (order + 1); }
+
+// Launch geometry of the owner-computes kernel, shared by the host partitioner (which sizes parts
+// to fit) and the kernel templates (which size their LDS from it).
+//   waves     wavefronts per workgroup
+//   maxBlocks BSR blocks a part may own (FP64 LDS accumulator, 32 B per block)
+//   maxRows   block rows a part may own
+struct OwnerConfig {
+  int waves;
+  int maxBlocks;
+  int maxRows;
+};
+constexpr OwnerConfig owner_config(int order) {
+  return order == 1   ? OwnerConfig{4, 2048, 512}
+         : order == 2 ? OwnerConfig{4, 1536, 256}
+         : order == 3 ? OwnerConfig{2, 1024, 128}
+                      : OwnerConfig{2, 1024, 64};
+}
+// Residual-only parts: only the per-row residual lives in LDS.
+constexpr int kResidualOnlyMaxRows = 1024;
+
+// Host-side result of partitioning a mesh into owner parts.
+struct PlanHost {
+  int order = 0;
+  int numNodes = 0;
+  int numElements = 0;
+  int numParts = 0;
+  int numColours = 1;
+  int maxRowLength = 0;
+  int64_t numBlocks = 0;
+  std::vector<int> conn;          // dense element-major [E*N]
+  std::vector<int> nodeLrow;      // [numNodes] index of the node within its part's row list
+  std::vector<int> partRowPtr;    // [P+1]
+  std::vector<int> rows;          // global row (node) ids, ascending within a part
+  std::vector<int> rowLbase;      // LDS block offset of each owned row within its part
+  std::vector<int> partBlocks;    // [P] blocks owned by the part
+  std::vector<int> partRunPtr;    // [P+1]
+  std::vector<int> runs;          // [R*4] (lds block start, global block start, blocks, 0)
+  std::vector<int> partElemPtr;   // [P+1]
+  std::vector<int> elems;         // element ids touching the part, grouped by colour
+  std::vector<int> colourOff;     // [P*(C+1)] offsets of each colour within a part's element list
+  std::vector<uint8_t> rank;      // [E*N*N] position of block (conn[e,i], conn[e,j]) in row conn[e,i]
+};
+
+int build_plan_host(int order, int numNodes, int numElements, const int *connPtr, const int *conn,
+                    const int *rowp, const int *cols, const double *nodeCoords, int flags, PlanHost &out);
+
+}  // namespace fem
+
+// Opaque plan handle (device-resident arrays built from a PlanHost).
+struct fem_plan {
+  int order = 0;
+  int numNodes = 0;
+  int numElements = 0;
+  int numParts = 0;
+  int numColours = 1;
+  int flags = 0;
+  int device = 0;
+  int64_t numBlocks = 0;
+  int64_t elementVisits = 0;
+  int64_t deviceBytes = 0;
+  int *dConn = nullptr;
+  int *dNodeLrow = nullptr;
+  int *dPartRowPtr = nullptr;
+  int *dRows = nullptr;
+  int *dRowLbase = nullptr;
+  int *dPartBlocks = nullptr;
+  int *dPartRunPtr = nullptr;
+  int *dRuns = nullptr;
+  int *dPartElemPtr = nullptr;
+  int *dElems = nullptr;
+  int *dColourOff = nullptr;
+  uint8_t *dRank = nullptr;
+};

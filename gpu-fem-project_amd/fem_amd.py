@@ -1,0 +1,340 @@
+"""ctypes binding of libfem_amd.so (the C ABI declared in include/fem_amd.h).
+
+Host helpers take and return numpy arrays; device entry points take torch tensors on a ROCm device
+(torch is only the device-memory and stream plumbing) and launch on torch's current stream.
+
+The library is the only compute path: if libfem_amd.so is missing, `load()` raises. There is no CPU
+fallback for the device entry points.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfem_amd.so")
+
+FEM_OK = 0
+FEM_PREC_FP64 = 0
+FEM_PREC_MIXED = 1
+FEM_PLAN_DETERMINISTIC = 1
+FEM_PLAN_RESIDUAL_ONLY = 2
+
+STATUS_NAMES = {
+    0: "FEM_OK",
+    -1: "FEM_ERR_INVALID_ORDER",
+    -2: "FEM_ERR_NULL_POINTER",
+    -3: "FEM_ERR_INVALID_ARGUMENT",
+    -4: "FEM_ERR_HIP",
+    -5: "FEM_ERR_MAP_MISS",
+    -6: "FEM_ERR_OUT_OF_MEMORY",
+    -7: "FEM_ERR_UNSUPPORTED",
+    -8: "FEM_ERR_BAD_GEOMETRY",
+    -9: "FEM_ERR_BUDGET",
+}
+
+# Every symbol include/fem_amd.h declares, with its ctypes signature.
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_I64 = ctypes.c_int64
+_U64 = ctypes.c_uint64
+_PI64 = ctypes.POINTER(ctypes.c_int64)
+SIGNATURES = {
+    "fem_last_error": ([], ctypes.c_char_p),
+    "fem_version": ([], _I),
+    "fem_basis_table": ([_I, _P, _P, _P, _P], _I),
+    "fem_square_mesh_size": ([_I, _I, _I, _PI64, _PI64], _I),
+    "fem_square_mesh": ([_I, _I, _I, _D, _U64, _P, _P], _I),
+    "fem_annulus_mesh_size": ([_I, _I, _I, _PI64, _PI64], _I),
+    "fem_annulus_mesh": ([_I, _I, _I, _D, _D, _D, _U64, _P, _P], _I),
+    "fem_fill_uniform": ([_I64, _D, _D, _U64, _P], _I),
+    "fem_fill_smooth_field": ([_I64, _P, _D, _U64, _D, _U64, _P], _I),
+    "fem_min_detj": ([_I, _I, _P, _P, _P, ctypes.POINTER(_D)], _I),
+    "fem_bsr_pattern_rowp": ([_I, _I, _I, _P, _P, _P], _I),
+    "fem_bsr_pattern_cols": ([_I, _I, _I, _P, _P, _P, _P], _I),
+    "fem_element_bcsr_map": ([_P, _P, _I, _I, _I, _P, _P, _I, _P], _I),
+    "fem_element_colouring": ([_I, _I, _I, _P, _P, _P, ctypes.POINTER(_I)], _I),
+    "fem_residual": ([_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _I, _P, ctypes.POINTER(_D)], _I),
+    "fem_jacobian": ([_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _P, _P, _I, _P, ctypes.POINTER(_D)], _I),
+    "fem_jacobian_coloured": (
+        [_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _P, _P, _I, _P, _P, _I, _P, ctypes.POINTER(_D)],
+        _I,
+    ),
+    "fem_plan_create": ([_I, _I, _I, _P, _P, _P, _P, _P, _I, ctypes.POINTER(_P)], _I),
+    "fem_plan_info": ([_P, _PI64, _PI64, _PI64, _PI64], _I),
+    "fem_plan_assemble": ([_P, _P, _P, _D, _D, _D, _P, _P, _I, _P, ctypes.POINTER(_D)], _I),
+    "fem_plan_destroy": ([_P], _I),
+    "fem_bsr_spmv": ([_I, _P, _P, _P, _P, _P, _P], _I),
+}
+
+_lib = None
+
+
+class FemError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libfem_amd.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(path)
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != FEM_OK:
+        msg = load().fem_last_error().decode()
+        raise FemError(status, msg)
+
+
+def _ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _dptr(t) -> Optional[int]:
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def nodes_per_element(order: int) -> int:
+    return (order + 1) ** 2
+
+
+# ---- host: element tables, meshes, fields --------------------------------------------------------
+def basis_table(order: int):
+    n = order + 1
+    phi = np.zeros(n * n)
+    dphi = np.zeros(n * n)
+    xg = np.zeros(n)
+    wg = np.zeros(n)
+    _check(load().fem_basis_table(order, _ptr(phi), _ptr(dphi), _ptr(xg), _ptr(wg)))
+    return phi.reshape(n, n), dphi.reshape(n, n), xg, wg
+
+
+@dataclass
+class Mesh:
+    order: int
+    coords: np.ndarray  # float64 [numNodes, 2]
+    conn: np.ndarray  # int32 [numElements, N]
+    kind: str = ""
+
+    @property
+    def num_nodes(self) -> int:
+        return int(self.coords.shape[0])
+
+    @property
+    def num_elements(self) -> int:
+        return int(self.conn.shape[0])
+
+    @property
+    def npe(self) -> int:
+        return nodes_per_element(self.order)
+
+    @property
+    def conn_ptr(self) -> np.ndarray:
+        return (np.arange(self.num_elements, dtype=np.int64) * self.npe).astype(np.int32)
+
+
+def square_mesh(n: int, order: int, jitter: float = 0.0, seed: int = 0, ny: Optional[int] = None) -> Mesh:
+    ny = n if ny is None else ny
+    lib = load()
+    nn = ctypes.c_int64()
+    ne = ctypes.c_int64()
+    _check(lib.fem_square_mesh_size(n, ny, order, ctypes.byref(nn), ctypes.byref(ne)))
+    coords = np.zeros((nn.value, 2))
+    conn = np.zeros((ne.value, nodes_per_element(order)), dtype=np.int32)
+    _check(lib.fem_square_mesh(n, ny, order, jitter, seed, _ptr(coords), _ptr(conn)))
+    return Mesh(order, coords, conn, f"square {n}x{ny} p={order}")
+
+
+def annulus_mesh(nr: int, ntheta: int, order: int, r_in: float = 1.0, r_out: float = 2.0, jitter: float = 0.0,
+                 seed: int = 0) -> Mesh:
+    lib = load()
+    nn = ctypes.c_int64()
+    ne = ctypes.c_int64()
+    _check(lib.fem_annulus_mesh_size(nr, ntheta, order, ctypes.byref(nn), ctypes.byref(ne)))
+    coords = np.zeros((nn.value, 2))
+    conn = np.zeros((ne.value, nodes_per_element(order)), dtype=np.int32)
+    _check(lib.fem_annulus_mesh(nr, ntheta, order, r_in, r_out, jitter, seed, _ptr(coords), _ptr(conn)))
+    return Mesh(order, coords, conn, f"annulus {nr}x{ntheta} p={order}")
+
+
+def fill_uniform(n: int, lo: float, hi: float, seed: int) -> np.ndarray:
+    out = np.zeros(n)
+    _check(load().fem_fill_uniform(n, lo, hi, seed, _ptr(out)))
+    return out
+
+
+def smooth_field(coords: np.ndarray, amplitude: float = 1e-2, phase_seed: int = 0, pert: float = 0.0,
+                 pert_seed: int = 0) -> np.ndarray:
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    out = np.zeros(coords.shape[0] * 2)
+    _check(load().fem_fill_smooth_field(coords.shape[0], _ptr(coords), amplitude, phase_seed, pert, pert_seed,
+                                        _ptr(out)))
+    return out
+
+
+def min_detj(mesh: Mesh) -> float:
+    v = ctypes.c_double()
+    cp = mesh.conn_ptr
+    _check(load().fem_min_detj(mesh.order, mesh.num_elements, _ptr(cp), _ptr(mesh.conn), _ptr(mesh.coords),
+                               ctypes.byref(v)))
+    return v.value
+
+
+# ---- host: pattern, map, colouring ---------------------------------------------------------------
+@dataclass
+class Pattern:
+    rowp: np.ndarray  # int32 [numNodes+1]
+    cols: np.ndarray  # int32 [nnzb]
+
+    @property
+    def num_blocks(self) -> int:
+        return int(self.rowp[-1])
+
+
+def bsr_pattern(mesh: Mesh) -> Pattern:
+    lib = load()
+    cp = mesh.conn_ptr
+    rowp = np.zeros(mesh.num_nodes + 1, dtype=np.int32)
+    _check(lib.fem_bsr_pattern_rowp(mesh.num_nodes, mesh.num_elements, mesh.npe, _ptr(cp), _ptr(mesh.conn),
+                                    _ptr(rowp)))
+    cols = np.zeros(int(rowp[-1]), dtype=np.int32)
+    _check(lib.fem_bsr_pattern_cols(mesh.num_nodes, mesh.num_elements, mesh.npe, _ptr(cp), _ptr(mesh.conn),
+                                    _ptr(rowp), _ptr(cols)))
+    return Pattern(rowp, cols)
+
+
+def element_bcsr_map(mesh: Mesh, pattern: Pattern, bsize: int = 2) -> np.ndarray:
+    cp = mesh.conn_ptr
+    out = np.zeros(mesh.num_elements * mesh.npe * mesh.npe, dtype=np.int32)
+    _check(load().fem_element_bcsr_map(_ptr(cp), _ptr(mesh.conn), mesh.num_elements, mesh.npe, mesh.num_nodes,
+                                       _ptr(pattern.rowp), _ptr(pattern.cols), bsize, _ptr(out)))
+    return out
+
+
+def element_colouring(mesh: Mesh):
+    cp = mesh.conn_ptr
+    colour = np.zeros(mesh.num_elements, dtype=np.int32)
+    nc = ctypes.c_int()
+    _check(load().fem_element_colouring(mesh.num_nodes, mesh.num_elements, mesh.npe, _ptr(cp), _ptr(mesh.conn),
+                                        _ptr(colour), ctypes.byref(nc)))
+    return colour, nc.value
+
+
+def colour_lists(colour: np.ndarray, num_colours: int):
+    """Element ids grouped by colour and the numColours+1 offsets."""
+    order = np.argsort(colour, kind="stable").astype(np.int32)
+    counts = np.bincount(colour, minlength=num_colours)
+    ptr = np.zeros(num_colours + 1, dtype=np.int32)
+    ptr[1:] = np.cumsum(counts)
+    return order, ptr
+
+
+# ---- device ----------------------------------------------------------------------------------------
+def _elapsed_arg(timed: bool):
+    return ctypes.byref(ctypes.c_double()) if timed else None
+
+
+def residual(order, conn_ptr, conn, num_elements, states, coords, E, nu, t, out, precision=FEM_PREC_FP64,
+             stream=None, timed=False):
+    """out += R (device tensors). Returns seconds when timed."""
+    sec = ctypes.c_double()
+    _check(load().fem_residual(order, _dptr(conn_ptr), _dptr(conn), num_elements, _dptr(states), _dptr(coords),
+                               E, nu, t, _dptr(out), precision, _stream(stream),
+                               ctypes.byref(sec) if timed else None))
+    return sec.value if timed else None
+
+
+def jacobian(order, conn_ptr, conn, num_elements, states, coords, E, nu, t, bcsr_map, res, values,
+             precision=FEM_PREC_FP64, stream=None, timed=False):
+    """values += K through the map; res (optional) += R."""
+    sec = ctypes.c_double()
+    _check(load().fem_jacobian(order, _dptr(conn_ptr), _dptr(conn), num_elements, _dptr(states), _dptr(coords), E,
+                               nu, t, _dptr(bcsr_map), _dptr(res), _dptr(values), precision, _stream(stream),
+                               ctypes.byref(sec) if timed else None))
+    return sec.value if timed else None
+
+
+def jacobian_coloured(order, conn_ptr, conn, num_elements, states, coords, E, nu, t, bcsr_map, colour_elems,
+                      colour_ptr: np.ndarray, res, values, precision=FEM_PREC_FP64, stream=None, timed=False):
+    sec = ctypes.c_double()
+    colour_ptr = np.ascontiguousarray(colour_ptr, dtype=np.int32)
+    _check(load().fem_jacobian_coloured(order, _dptr(conn_ptr), _dptr(conn), num_elements, _dptr(states),
+                                        _dptr(coords), E, nu, t, _dptr(bcsr_map), _dptr(colour_elems),
+                                        _ptr(colour_ptr), len(colour_ptr) - 1, _dptr(res), _dptr(values), precision,
+                                        _stream(stream), ctypes.byref(sec) if timed else None))
+    return sec.value if timed else None
+
+
+def bsr_spmv(pattern_rowp, pattern_cols, values, x, y, stream=None):
+    n = pattern_rowp.numel() - 1
+    _check(load().fem_bsr_spmv(n, _dptr(pattern_rowp), _dptr(pattern_cols), _dptr(values), _dptr(x), _dptr(y),
+                               _stream(stream)))
+
+
+class Plan:
+    """Resident owner-computes assembly plan (fem_plan_*)."""
+
+    def __init__(self, mesh: Mesh, pattern: Pattern, flags: int = 0, morton: bool = True):
+        self._lib = load()
+        handle = ctypes.c_void_p()
+        cp = mesh.conn_ptr
+        _check(self._lib.fem_plan_create(mesh.order, mesh.num_nodes, mesh.num_elements, _ptr(cp), _ptr(mesh.conn),
+                                         _ptr(pattern.rowp), _ptr(pattern.cols),
+                                         _ptr(mesh.coords) if morton else None, flags, ctypes.byref(handle)))
+        self.handle = handle
+        self.order = mesh.order
+        self.flags = flags
+
+    def info(self):
+        vals = [ctypes.c_int64() for _ in range(4)]
+        _check(self._lib.fem_plan_info(self.handle, *[ctypes.byref(v) for v in vals]))
+        return dict(zip(["parts", "blocks", "element_visits", "device_bytes"], [v.value for v in vals]))
+
+    def assemble(self, states, coords, E, nu, t, res=None, values=None, precision=FEM_PREC_FP64, stream=None,
+                 timed=False):
+        sec = ctypes.c_double()
+        _check(self._lib.fem_plan_assemble(self.handle, _dptr(states), _dptr(coords), E, nu, t, _dptr(res),
+                                           _dptr(values), precision, _stream(stream),
+                                           ctypes.byref(sec) if timed else None))
+        return sec.value if timed else None
+
+    def close(self):
+        if self.handle:
+            self._lib.fem_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

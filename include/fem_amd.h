@@ -1,0 +1,163 @@
+/*
+ * C ABI of the MI355X plane-stress assembly library (libfem_amd.so).
+ *
+ * Element formulation: 2-D linear-elastic plane stress (LINEAR strain) on tensor-product Lagrange
+ * quadrilaterals of order 1..4 (N = (order+1)^2 nodes, (order+1)^2 Gauss points), 2 states per node.
+ *
+ * Data layout (identical to the reference's, element/FEKernels.h and verification/TestKernels.cpp):
+ *   nodeCoords  double [numNodes*2]   (x, y) interleaved        (TestKernels.cpp:175-179)
+ *   states      double [numNodes*2]   (u, v) interleaved        (TestKernels.cpp:181-187)
+ *   residual    double [numNodes*2]   node-interleaved          (FEKernels.h:44-61)
+ *   connPtr     int    [numElements]  element e's nodes are conn[connPtr[e] + a], a = ay*(order+1)+ax
+ *   conn        int    [numElements*N]
+ *   BSR matrix  TACS BCSRMatData: rowp[numNodes+1], cols[nnzb] (sorted per row, diagonal present),
+ *               values double[nnzb*4], 2x2 blocks stored row-major (TACSHelpers.h:248-307)
+ *   elementBCSRMap int [numElements*N*N]: map[e*N*N + i*N + j] = 4 * (BSR slot of block
+ *               (conn[e,i], conn[e,j]))  (TACSHelpers.h:132-182)
+ *
+ * Conventions: every function returns an fem_status code (0 = success, negative = error) and never
+ * exits the process; fem_last_error() gives a message for the calling thread's last failure.
+ * "device" pointers are HIP device memory owned by the caller; "host" pointers are host memory.
+ * `stream` is a hipStream_t passed as void* (NULL = the default stream). Device entry points are
+ * asynchronous on `stream` unless `elapsedSeconds` is non-NULL, in which case the call records HIP
+ * events around its launches, waits for them and stores the elapsed time in seconds.
+ */
+#ifndef FEM_AMD_H
+#define FEM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes and modes ------------------------------------------------------------------ */
+enum fem_status {
+  FEM_OK = 0,
+  FEM_ERR_INVALID_ORDER = -1,    /* element order outside 1..4 (the reference silently no-ops) */
+  FEM_ERR_NULL_POINTER = -2,     /* a required pointer is NULL */
+  FEM_ERR_INVALID_ARGUMENT = -3, /* sizes, node ids or parameters out of range */
+  FEM_ERR_HIP = -4,              /* a HIP runtime call failed */
+  FEM_ERR_MAP_MISS = -5,         /* an element node pair has no block in the BSR pattern */
+  FEM_ERR_OUT_OF_MEMORY = -6,
+  FEM_ERR_UNSUPPORTED = -7,      /* valid input that this build does not handle (message says why) */
+  FEM_ERR_BAD_GEOMETRY = -8,     /* generated mesh has det(J) <= 0 at a quadrature point */
+  FEM_ERR_BUDGET = -9            /* serving admission: request exceeds the device-memory budget */
+};
+
+enum fem_precision {
+  FEM_PREC_FP64 = 0, /* all arithmetic in FP64 */
+  FEM_PREC_MIXED = 1 /* basis, stress and B^T D B in FP32; det(J), J^-1 and all accumulation in FP64 */
+};
+
+/* Message describing the calling thread's most recent failure ("" if none). */
+const char *fem_last_error(void);
+/* Library version as major*10000 + minor*100 + patch. */
+int fem_version(void);
+
+/* ---- element primitives (host) ----------------------------------------------------------------
+ * Tabulate the 1-D basis of `order` at the order+1 Gauss points: phi[k*(order+1)+i] = N_k(xi_i),
+ * dphi[...] = dN_k/dxi(xi_i), and the Gauss points/weights. Same values as lagrangePoly1dDeriv and
+ * getGaussQuad* in LagrangeShapeFuncs.h / GaussQuadrature.h (LagrangeShapeFuncs.h:129-216,
+ * GaussQuadrature.h:46-99 of the reference). */
+int fem_basis_table(int order, double *phi, double *dphi, double *gaussPoints, double *gaussWeights);
+
+/* ---- synthetic meshes and fields (host) --------------------------------------------------------
+ * Structured meshes built the way the reference's gmsh meshes are consumed (TACS tensor node order,
+ * x fastest), with nodes at the cosine knots. Random draws use SplitMix64; u01 = (x >> 11) * 2^-53. */
+int fem_square_mesh_size(int nx, int ny, int order, int64_t *numNodes, int64_t *numElements);
+/* Unit square, nx*ny elements, node id = jy*(order*nx+1)+jx. Interior nodes are moved by
+ * U(-jitter, jitter) times the smaller adjacent knot gap along each axis (draws in node-id order,
+ * dx then dy). conn is element-major, element e = ey*nx+ex. */
+int fem_square_mesh(int nx, int ny, int order, double jitter, uint64_t seed, double *nodeCoords, int *conn);
+int fem_annulus_mesh_size(int nr, int ntheta, int order, int64_t *numNodes, int64_t *numElements);
+/* Annulus rIn <= r <= rOut, nr radial x ntheta circumferential elements with a periodic seam.
+ * xi runs radially, eta circumferentially; node id = jtheta*(order*nr+1)+jr; element e = etheta*nr+er.
+ * Radially interior nodes are jittered in (r, theta) like the square mesh. */
+int fem_annulus_mesh(int nr, int ntheta, int order, double rIn, double rOut, double jitter, uint64_t seed,
+                     double *nodeCoords, int *conn);
+/* out[i] = U(lo, hi), i = 0..n-1, in order. */
+int fem_fill_uniform(int64_t n, double lo, double hi, uint64_t seed, double *out);
+/* Smooth displacement field u = A(sin(2 pi x + p1) + sin(2 pi y + p2)), v = A(cos(2 pi x + p3) +
+ * cos(2 pi y + p4)) with phases p1..p4 ~ U(0, 2 pi) drawn from phaseSeed (all zero when phaseSeed
+ * is 0: the reference's field, TACSHelpers.h:192-196), plus U(-pert, pert) per state from pertSeed. */
+int fem_fill_smooth_field(int64_t numNodes, const double *nodeCoords, double amplitude, uint64_t phaseSeed,
+                          double pert, uint64_t pertSeed, double *states);
+/* Minimum det(J) over all Gauss points of all elements (host, FP64). */
+int fem_min_detj(int order, int numElements, const int *connPtr, const int *conn, const double *nodeCoords,
+                 double *minDetJ);
+
+/* ---- BSR pattern, element map, colouring (host) ------------------------------------------------
+ * TACS-style block pattern of the mesh: block (i, j) exists iff nodes i and j share an element;
+ * columns sorted ascending per row, diagonal included. Call _rowp first (fills numNodes+1 entries,
+ * rowp[numNodes] = number of blocks), then _cols with a cols array of that length. */
+int fem_bsr_pattern_rowp(int numNodes, int numElements, int nodesPerElement, const int *connPtr, const int *conn,
+                         int *rowp);
+int fem_bsr_pattern_cols(int numNodes, int numElements, int nodesPerElement, const int *connPtr, const int *conn,
+                         const int *rowp, int *cols);
+/* TACS-free overload of generateElementBCSRMap (verification/TACSHelpers.h:138-182): same output
+ * layout, map[e*N*N + i*N + j] = slot * bsize * bsize. A missing block is an error
+ * (FEM_ERR_MAP_MISS), not a printed warning with the entry left unset (TACSHelpers.h:176-178). */
+int fem_element_bcsr_map(const int *connPtr, const int *conn, int numElements, int nodesPerElement, int numRows,
+                         const int *rowp, const int *cols, int bsize, int *map);
+/* Greedy element colouring: no two elements of one colour share a node. colour[e] in
+ * 0..*numColours-1, assigned in element order. */
+int fem_element_colouring(int numNodes, int numElements, int nodesPerElement, const int *connPtr, const int *conn,
+                          int *colour, int *numColours);
+
+/* ---- drop-in device entry points ---------------------------------------------------------------
+ * Same arguments and accumulate (+=) semantics as the reference's dispatchers, plus precision,
+ * stream and timing. The caller zeroes the outputs first (TestKernels.cpp:237,241).
+ *
+ * fem_residual replaces runResidualKernel (element/FEKernels.h:1030-1102): residual += R(states).
+ * fem_jacobian replaces runJacobianKernel (element/FEKernels.h:1125-1225): matEntries += K through
+ *   elementBCSRMap, and, when residual is non-NULL, residual += R. FP64 atomic scatter. */
+int fem_residual(int order, const int *connPtr, const int *conn, int numElements, const double *states,
+                 const double *nodeCoords, double E, double nu, double t, double *residual, int precision,
+                 void *stream, double *elapsedSeconds);
+int fem_jacobian(int order, const int *connPtr, const int *conn, int numElements, const double *states,
+                 const double *nodeCoords, double E, double nu, double t, const int *elementBCSRMap,
+                 double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds);
+/* Deterministic element-colouring variant of fem_jacobian (no reference counterpart beyond the
+ * element-colouring requirement of the same path): elements are processed colour by colour from
+ * colourElements (device, element ids grouped by colour, colourPtr (host) gives numColours+1
+ * offsets); sums use plain read-add-write, so results are bitwise reproducible. */
+int fem_jacobian_coloured(int order, const int *connPtr, const int *conn, int numElements, const double *states,
+                          const double *nodeCoords, double E, double nu, double t, const int *elementBCSRMap,
+                          const int *colourElements, const int *colourPtr, int numColours, double *residual,
+                          double *matEntries, int precision, void *stream, double *elapsedSeconds);
+
+/* ---- resident assembly plan (owner-computes, write-once) ---------------------------------------
+ * Built once per mesh from host arrays, kept on the device. Nodes are split into parts whose BSR
+ * rows fit a workgroup's LDS; each workgroup assembles every element touching its rows and writes
+ * its residual entries and BSR blocks exactly once (no atomics to HBM, no zero-fill). Results are
+ * written (not accumulated) into the caller's residual / values in the layout of `rowp`/`cols`.
+ * nodeCoordsHost (optional) orders nodes along a Morton curve so parts are compact. */
+typedef struct fem_plan fem_plan;
+
+int fem_plan_create(int order, int numNodes, int numElements, const int *connPtr, const int *conn,
+                    const int *rowp, const int *cols, const double *nodeCoordsHost, int flags, fem_plan **plan);
+/* flags for fem_plan_create */
+#define FEM_PLAN_DETERMINISTIC 1 /* colour-phased LDS sums: bitwise reproducible results */
+#define FEM_PLAN_RESIDUAL_ONLY 2 /* size parts for residual-only assembly */
+
+/* What a plan holds: number of parts, BSR blocks, element visits (elements counted once per part
+ * they touch), resident device bytes. */
+int fem_plan_info(const fem_plan *plan, int64_t *numParts, int64_t *numBlocks, int64_t *elementVisits,
+                  int64_t *deviceBytes);
+/* residual (may be NULL) and matEntries (may be NULL) are overwritten. At least one must be given. */
+int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCoords, double E, double nu, double t,
+                      double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds);
+int fem_plan_destroy(fem_plan *plan);
+
+/* ---- BSR utilities (device) -------------------------------------------------------------------
+ * y = A x for the 2x2-block BSR matrix (used to check R = K u). */
+int fem_bsr_spmv(int numRows, const int *rowp, const int *cols, const double *values, const double *x, double *y,
+                 void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEM_AMD_H */
