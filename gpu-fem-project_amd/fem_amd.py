@@ -89,6 +89,13 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         return _lib
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    # One HIP runtime per process: torch bundles its own libamdhip64 (SONAME libamdhip64.so.7). Loading
+    # torch first makes this library's libamdhip64.so.7 dependency resolve to that same copy instead of
+    # pulling a second runtime from /opt/rocm, which would not share devices or streams with torch.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     for name, (args, res) in SIGNATURES.items():
         fn = getattr(lib, name)
