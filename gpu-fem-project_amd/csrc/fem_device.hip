@@ -11,8 +11,12 @@
 //
 // Work mapping (all paths): a wavefront processes EPW = floor(64/N) elements at once; lane
 // (el, idx) = (lane / N, lane % N). idx is a node in the gather, a quadrature point in the geometry
-// stage and a column node b in the gradient and stiffness stages. Per-wave LDS holds the gathered
-// element nodes, per-quadrature-point geometry and the weighted gradients s_q*gradN_a(q).
+// stage and a column node b in the stiffness stage. The stiffness stage runs quadrature-point-outer:
+// at each q every lane forms s_q*gradN_b(q), publishes it in a double-buffered per-wave LDS slot, and
+// accumulates the Gram sums of its column b against every row a of a chunk of rows. Per-wave LDS
+// therefore holds only the gathered nodes, the per-q geometry and one q's weighted gradients.
+// Per-element LDS strides are padded to an odd number of 16-byte units so the broadcast reads of the
+// elements of one wavefront fall in distinct banks.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -32,6 +36,13 @@ namespace fem {
     }                                                                                                     \
   } while (0)
 
+// Smallest count >= n of elements of `bytes` bytes spanning an odd number of 16-byte units.
+constexpr int odd_units(int n, int bytes) {
+  int s = n;
+  while ((s * bytes) % 16 != 0 || ((s * bytes) / 16) % 2 == 0) s++;
+  return s;
+}
+
 template <int P>
 struct Ord {
   static constexpr int NP = P + 1;
@@ -39,6 +50,11 @@ struct Ord {
   static constexpr int Q = N;
   static constexpr int EPW = 64 / N;
   static constexpr int LANES = EPW * N;
+  static constexpr int RPAD = rank_pad(P);                  // rank bytes stored per (element, column node)
+  static constexpr int AC = P <= 3 ? N : 13;  // rows per stiffness chunk
+  static constexpr int NCHUNK = (N + AC - 1) / AC;
+  static constexpr int NODAL_STRIDE = 2 * odd_units(2 * N, 16);  // doubles; node a at [4a, 4a+4)
+  static constexpr int GEO_STRIDE = 2 * odd_units(4 * Q, 16);    // doubles; point q at [8q, 8q+8)
 };
 
 // 1-D basis at the Gauss points: phi[k][i] = N_k(xi_i), dphi[k][i] = dN_k/dxi(xi_i), w[i].
@@ -67,10 +83,12 @@ struct Vec2<float> {
 template <int P, typename T>
 struct __align__(16) WaveLDS {
   using O = Ord<P>;
-  typename Vec2<T>::type sg[O::EPW][O::Q][O::N];  // s_q * gradN_a(q)
-  double nodal[O::EPW][O::N][4];                   // x, y, u, v of each element node
-  double geo[O::EPW][O::Q][8];                     // J^-1 (row-major), s_q, s_q*sigma_xx, s_q*sigma_yy, s_q*sigma_xy
-  int own[O::EPW][O::N];                           // owner path: local row of node a in this part, or -1
+  using V2 = typename Vec2<T>::type;
+  static constexpr int SG_STRIDE = odd_units(O::N, int(sizeof(V2)));
+  double nodal[O::EPW][O::NODAL_STRIDE];  // x, y, u, v of each element node
+  double geo[O::EPW][O::GEO_STRIDE];      // per q: J^-1 (row-major), s_q, s_q*sigma_xx, s_q*sigma_yy, s_q*sigma_xy
+  V2 sg[2][O::EPW][SG_STRIDE];            // s_q * gradN_b(q) of the current quadrature point (double-buffered)
+  int own[O::EPW][O::N];                  // owner path: local row of node a in this part, or -1
 };
 
 __device__ __forceinline__ void wave_sync() { __syncwarp(); }
@@ -80,6 +98,23 @@ __device__ __forceinline__ void copy_basis(Basis1D &dst, const Basis1D &src) {
   double *d = reinterpret_cast<double *>(&dst);
   constexpr int n = int(sizeof(Basis1D) / sizeof(double));
   for (int i = threadIdx.x; i < n; i += blockDim.x) d[i] = s[i];
+}
+
+// Reference gradients g_b(q) = (dN_b/dxi, dN_b/deta) at every quadrature point, gref[q*N + b].
+template <int P>
+__device__ __forceinline__ void fill_gref(double2 *gref, const Basis1D &src) {
+  constexpr int NP = Ord<P>::NP, N = Ord<P>::N, Q = Ord<P>::Q;
+  for (int i = threadIdx.x; i < Q * N; i += blockDim.x) {
+    const int q = i / N, b = i - q * N;
+    const int qy = q / NP, qx = q - qy * NP, by = b / NP, bx = b - by * NP;
+    gref[i] = make_double2(src.phi[by][qy] * src.dphi[bx][qx], src.phi[bx][qx] * src.dphi[by][qy]);
+  }
+}
+
+template <int P, typename T>
+__device__ __forceinline__ void store_nodal(WaveLDS<P, T> &W, int el, int a, double2 xy, double2 uv) {
+  *reinterpret_cast<double2 *>(&W.nodal[el][4 * a]) = xy;
+  *reinterpret_cast<double2 *>(&W.nodal[el][4 * a + 2]) = uv;
 }
 
 // Geometry and stress at quadrature point q of element `el` (lane (el, q)).
@@ -104,8 +139,8 @@ __device__ __forceinline__ void stage_geometry(WaveLDS<P, T> &W, const Basis1D &
       const int a = ay * NP + ax;
       const double gx = py[ay] * dpx[ax];
       const double gy = px[ax] * dpy[ay];
-      const double2 xy = *reinterpret_cast<const double2 *>(&W.nodal[el][a][0]);
-      const double2 uv = *reinterpret_cast<const double2 *>(&W.nodal[el][a][2]);
+      const double2 xy = *reinterpret_cast<const double2 *>(&W.nodal[el][4 * a]);
+      const double2 uv = *reinterpret_cast<const double2 *>(&W.nodal[el][4 * a + 2]);
       J00 = fma(xy.x, gx, J00);
       J01 = fma(xy.x, gy, J01);
       J10 = fma(xy.y, gx, J10);
@@ -132,77 +167,94 @@ __device__ __forceinline__ void stage_geometry(WaveLDS<P, T> &W, const Basis1D &
   const T sxy = mu * (G01 + G10);
   const double s = (B.w[qx] * B.w[qy]) * detJ * m.t;
   const T ts = T(s);
-  double *g = W.geo[el][q];
+  double *g = &W.geo[el][8 * q];
   *reinterpret_cast<double2 *>(g + 0) = make_double2(i00, i01);
   *reinterpret_cast<double2 *>(g + 2) = make_double2(i10, i11);
   *reinterpret_cast<double2 *>(g + 4) = make_double2(s, double(ts * sxx));
   *reinterpret_cast<double2 *>(g + 6) = make_double2(double(ts * syy), double(ts * sxy));
 }
 
-// Physical gradients of basis function b at every quadrature point (lane (el, b)); residual entries
-// of node b; writes s_q * gradN_b(q) to LDS for the stiffness stage.
-template <int P, typename T>
-__device__ __forceinline__ void stage_gradients(WaveLDS<P, T> &W, const Basis1D &B, int el, int b,
-                                                T (&gN)[Ord<P>::Q][2], T &Rx, T &Ry) {
-  constexpr int NP = Ord<P>::NP;
-  const int bx = b % NP, by = b / NP;
-  T pbx[NP], dpbx[NP], pby[NP], dpby[NP];
-#pragma unroll
-  for (int i = 0; i < NP; i++) {
-    pbx[i] = T(B.phi[bx][i]);
-    dpbx[i] = T(B.dphi[bx][i]);
-    pby[i] = T(B.phi[by][i]);
-    dpby[i] = T(B.dphi[by][i]);
-  }
+// Residual entries of node b and (JAC) the stiffness blocks K(a, b) of every row a, for lane (el, b).
+// emit(a, k) receives each block, row-major [xx, xy, yx, yy], in FP64. Every lane of the wavefront
+// must call this (it synchronises the wavefront); `valid` masks the lane's LDS writes and emits.
+// The geometry of quadrature point q+1 is read while the Gram sums of q are accumulated.
+template <int P, typename T, bool JAC, typename Emit>
+__device__ __forceinline__ void stage_stiffness(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el,
+                                                int b, bool valid, T &Rx, T &Ry, Emit &&emit) {
+  using O = Ord<P>;
+  constexpr int AC = O::AC, N = O::N, Q = O::Q;
+  using V2 = typename Vec2<T>::type;
+  const T mu = T(m.mu), lam = T(m.lambda), c1 = lam + 2 * mu;
   Rx = 0;
   Ry = 0;
+  int buf = 0;
 #pragma unroll
-  for (int qy = 0; qy < NP; qy++) {
+  for (int c = 0; c < (JAC ? O::NCHUNK : 1); c++) {
+    const int a0 = c * AC;
+    T S[AC][4];
 #pragma unroll
-    for (int qx = 0; qx < NP; qx++) {
-      const int q = qy * NP + qx;
-      const T gx = pby[qy] * dpbx[qx];
-      const T gy = pbx[qx] * dpby[qy];
-      const double *g = W.geo[el][q];
-      const double2 ia = *reinterpret_cast<const double2 *>(g + 0);
-      const double2 ib = *reinterpret_cast<const double2 *>(g + 2);
-      const double2 sc = *reinterpret_cast<const double2 *>(g + 4);
-      const double2 sd = *reinterpret_cast<const double2 *>(g + 6);
+    for (int i = 0; i < AC; i++) S[i][0] = S[i][1] = S[i][2] = S[i][3] = 0;
+    // quadrature point 0
+    double2 gg = gref[b];
+    double2 ia = *reinterpret_cast<const double2 *>(&W.geo[el][0]);
+    double2 ib = *reinterpret_cast<const double2 *>(&W.geo[el][2]);
+    double2 sc = *reinterpret_cast<const double2 *>(&W.geo[el][4]);
+    double2 sd = *reinterpret_cast<const double2 *>(&W.geo[el][6]);
+#pragma unroll 1
+    for (int q = 0; q < Q; q++) {
+      const T gx = T(gg.x), gy = T(gg.y);
       const T nx = gx * T(ia.x) + gy * T(ib.x);
       const T ny = gx * T(ia.y) + gy * T(ib.y);
-      gN[q][0] = nx;
-      gN[q][1] = ny;
-      const T txx = T(sc.y), tyy = T(sd.x), txy = T(sd.y);
-      Rx += txx * nx + txy * ny;
-      Ry += txy * nx + tyy * ny;
-      const T s = T(sc.x);
-      typename Vec2<T>::type v;
-      v.x = s * nx;
-      v.y = s * ny;
-      W.sg[el][q][b] = v;
+      if (c == 0) {
+        const T txx = T(sc.y), tyy = T(sd.x), txy = T(sd.y);
+        Rx += txx * nx + txy * ny;
+        Ry += txy * nx + tyy * ny;
+      }
+      V2 v;
+      if (JAC) {
+        const T s = T(sc.x);
+        v.x = s * nx;
+        v.y = s * ny;
+      }
+      // next quadrature point's geometry (clamped: the last iteration re-reads point Q-1)
+      const int qn = q + 1 < Q ? q + 1 : q;
+      const double *g = &W.geo[el][8 * qn];
+      gg = gref[qn * N + b];
+      ia = *reinterpret_cast<const double2 *>(g + 0);
+      ib = *reinterpret_cast<const double2 *>(g + 2);
+      sc = *reinterpret_cast<const double2 *>(g + 4);
+      if (c == 0) sd = *reinterpret_cast<const double2 *>(g + 6);
+      if (JAC) {
+        if (valid) W.sg[buf][el][b] = v;
+        wave_sync();
+        const V2 *row = &W.sg[buf][el][a0];
+#pragma unroll
+        for (int i = 0; i < AC; i++) {
+          if (a0 + i < N) {
+            const V2 w = row[i];
+            S[i][0] = fma(w.x, nx, S[i][0]);
+            S[i][1] = fma(w.y, ny, S[i][1]);
+            S[i][2] = fma(w.x, ny, S[i][2]);
+            S[i][3] = fma(w.y, nx, S[i][3]);
+          }
+        }
+        buf ^= 1;
+      }
+    }
+    if (JAC) {
+#pragma unroll
+      for (int i = 0; i < AC; i++) {
+        if (a0 + i < N) {
+          double k[4];
+          k[0] = double(c1 * S[i][0] + mu * S[i][1]);
+          k[1] = double(lam * S[i][2] + mu * S[i][3]);
+          k[2] = double(lam * S[i][3] + mu * S[i][2]);
+          k[3] = double(c1 * S[i][1] + mu * S[i][0]);
+          if (valid) emit(a0 + i, k);
+        }
+      }
     }
   }
-}
-
-// 2x2 stiffness block K(a, b) for lane (el, b), row-major [xx, xy, yx, yy].
-template <int P, typename T>
-__device__ __forceinline__ void stage_block(const WaveLDS<P, T> &W, const Material &m, int el, int a,
-                                            const T (&gN)[Ord<P>::Q][2], double (&k)[4]) {
-  constexpr int Q = Ord<P>::Q;
-  T Sxx = 0, Syy = 0, Sxy = 0, Syx = 0;
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const typename Vec2<T>::type v = W.sg[el][q][a];
-    Sxx = fma(v.x, gN[q][0], Sxx);
-    Syy = fma(v.y, gN[q][1], Syy);
-    Sxy = fma(v.x, gN[q][1], Sxy);
-    Syx = fma(v.y, gN[q][0], Syx);
-  }
-  const T mu = T(m.mu), lam = T(m.lambda), c1 = lam + 2 * mu;
-  k[0] = double(c1 * Sxx + mu * Syy);
-  k[1] = double(lam * Sxy + mu * Syx);
-  k[2] = double(lam * Syx + mu * Sxy);
-  k[3] = double(c1 * Syy + mu * Sxx);
 }
 
 // ---- element-parallel kernel (drop-in atomic path and colour path) -----------------------------
@@ -227,7 +279,9 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   constexpr int N = O::N;
   __shared__ WaveLDS<P, T> wl[WAVES];
   __shared__ Basis1D basis;
+  __shared__ double2 gref[O::Q * O::N];
   copy_basis(basis, args.basis);
+  fill_gref<P>(gref, args.basis);
   __syncthreads();
 
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -239,17 +293,24 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   if (valid) {
     eid = args.elemList ? args.elemList[slot] : args.first + int(slot);
     node = args.conn[args.connPtr[eid] + idx];
-    const double2 xy = *reinterpret_cast<const double2 *>(args.coords + 2 * int64_t(node));
-    const double2 uv = *reinterpret_cast<const double2 *>(args.states + 2 * int64_t(node));
-    *reinterpret_cast<double2 *>(&W.nodal[el][idx][0]) = xy;
-    *reinterpret_cast<double2 *>(&W.nodal[el][idx][2]) = uv;
+    store_nodal(W, el, idx, *reinterpret_cast<const double2 *>(args.coords + 2 * int64_t(node)),
+                *reinterpret_cast<const double2 *>(args.states + 2 * int64_t(node)));
   }
   wave_sync();
   if (valid) stage_geometry<P, T>(W, basis, args.mat, el, idx);
   wave_sync();
-  T gN[O::Q][2];
-  T Rx = 0, Ry = 0;
-  if (valid) stage_gradients<P, T>(W, basis, el, idx, gN, Rx, Ry);
+  const int *mrow = args.map + int64_t(eid) * N * N + idx;
+  T Rx, Ry;
+  stage_stiffness<P, T, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, [&](int a, const double(&k)[4]) {
+    double *blk = args.values + mrow[a * N];
+    if (ATOMIC) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) unsafeAtomicAdd(blk + c, k[c]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; c++) blk[c] += k[c];
+    }
+  });
   if (RES && valid) {
     double *r = args.residual + 2 * int64_t(node);
     if (ATOMIC) {
@@ -260,29 +321,12 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
       r[1] += double(Ry);
     }
   }
-  if (JAC) {
-    wave_sync();
-    if (valid) {
-      const int *mrow = args.map + int64_t(eid) * N * N + idx;
-      for (int a = 0; a < N; a++) {
-        double k[4];
-        stage_block<P, T>(W, args.mat, el, a, gN, k);
-        double *blk = args.values + mrow[a * N];
-        if (ATOMIC) {
-#pragma unroll
-          for (int c = 0; c < 4; c++) unsafeAtomicAdd(blk + c, k[c]);
-        } else {
-#pragma unroll
-          for (int c = 0; c < 4; c++) blk[c] += k[c];
-        }
-      }
-    }
-  }
 }
 
 // ---- owner-computes kernel (resident plan, write-once) --------------------------------------------
 struct OwnerArgs {
-  const int *conn;  // dense [E*N]
+  const int *visitConn;  // [visits*N] nodes of each element visit, parts' visits contiguous
+  const int *visitEid;   // [visits] element id of each visit
   const int *nodeLrow;
   const int *partRowPtr;
   const int *rows;
@@ -291,9 +335,8 @@ struct OwnerArgs {
   const int *partRunPtr;
   const int4 *runs;
   const int *partElemPtr;
-  const int *elems;
   const int *colourOff;
-  const uint8_t *rank;
+  const uint32_t *rank;  // [E][N][RPAD] bytes: rank[e][b][a] = slot of column conn[e,b] within row conn[e,a]
   int numParts;
   int numColours;
   const double *states;
@@ -304,31 +347,63 @@ struct OwnerArgs {
   Material mat;
 };
 
-template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC, bool DET>
+// Per-lane gathered inputs of one element visit.
+template <int P>
+struct Gather {
+  int node;
+  int eid;
+  int lrow;
+  double2 xy, uv;
+  uint32_t rk[Ord<P>::RPAD / 4];
+};
+
+// Level 1: node id and element id of visit `visit` (always in bounds: callers clamp the visit).
+template <int P>
+__device__ __forceinline__ void gather_level1(const OwnerArgs &a, int visit, int idx, Gather<P> &g) {
+  g.node = a.visitConn[int64_t(visit) * Ord<P>::N + idx];
+  g.eid = a.visitEid[visit];
+}
+
+// Level 2: coordinates, states, owner row and column ranks of the node/element of level 1.
+template <int P>
+__device__ __forceinline__ void gather_level2(const OwnerArgs &a, int idx, Gather<P> &g) {
+  g.xy = *reinterpret_cast<const double2 *>(a.coords + 2 * int64_t(g.node));
+  g.uv = *reinterpret_cast<const double2 *>(a.states + 2 * int64_t(g.node));
+  g.lrow = a.nodeLrow[g.node];
+  constexpr int W = Ord<P>::RPAD / 4;
+  const uint32_t *r = a.rank + (int64_t(g.eid) * Ord<P>::N + idx) * W;
+#pragma unroll
+  for (int i = 0; i < W; i++) g.rk[i] = r[i];
+}
+
+template <int P, typename T, bool RES, bool JAC, bool DET>
 __device__ __forceinline__ void owner_batch(const OwnerArgs &args, WaveLDS<P, T> &W, const Basis1D &basis,
-                                            double *acc, double *racc, const int *lrows, const int *lbase, int nr,
-                                            int e0, int bstart, int bend, int el, int idx, int lane) {
-  using O = Ord<P>;
-  constexpr int N = O::N;
-  const bool valid = lane < O::LANES && bstart + el < bend;
-  int eid = 0, own = -1;
+                                            const double2 *gref, double *acc, double *racc, const int *lrows,
+                                            const int *lbase, int nr, const Gather<P> &g, bool valid, int el,
+                                            int idx) {
+  int own = -1;
   if (valid) {
-    eid = args.elems[e0 + bstart + el];
-    const int node = args.conn[int64_t(eid) * N + idx];
-    const int lr = args.nodeLrow[node];
-    own = (lr < nr && lrows[lr] == node) ? lr : -1;
-    const double2 xy = *reinterpret_cast<const double2 *>(args.coords + 2 * int64_t(node));
-    const double2 uv = *reinterpret_cast<const double2 *>(args.states + 2 * int64_t(node));
-    *reinterpret_cast<double2 *>(&W.nodal[el][idx][0]) = xy;
-    *reinterpret_cast<double2 *>(&W.nodal[el][idx][2]) = uv;
+    own = (g.lrow < nr && lrows[g.lrow] == g.node) ? g.lrow : -1;
+    store_nodal(W, el, idx, g.xy, g.uv);
     W.own[el][idx] = own;
   }
   wave_sync();
   if (valid) stage_geometry<P, T>(W, basis, args.mat, el, idx);
   wave_sync();
-  T gN[O::Q][2];
-  T Rx = 0, Ry = 0;
-  if (valid) stage_gradients<P, T>(W, basis, el, idx, gN, Rx, Ry);
+  T Rx, Ry;
+  stage_stiffness<P, T, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, [&](int a, const double(&k)[4]) {
+    const int ownA = W.own[el][a];
+    if (ownA < 0) return;
+    const int r = int((g.rk[a >> 2] >> (8 * (a & 3))) & 0xFFu);
+    double *dst = acc + 4 * (lbase[ownA] + r);
+    if (DET) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) dst[c] += k[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; c++) atomicAdd(dst + c, k[c]);
+    }
+  });
   if (RES && valid && own >= 0) {
     if (DET) {
       racc[2 * own] += double(Rx);
@@ -336,26 +411,6 @@ __device__ __forceinline__ void owner_batch(const OwnerArgs &args, WaveLDS<P, T>
     } else {
       atomicAdd(&racc[2 * own], double(Rx));
       atomicAdd(&racc[2 * own + 1], double(Ry));
-    }
-  }
-  if (JAC) {
-    wave_sync();
-    if (valid) {
-      const uint8_t *rk = args.rank + int64_t(eid) * N * N + idx;
-      for (int a = 0; a < N; a++) {
-        const int ownA = W.own[el][a];
-        if (ownA < 0) continue;
-        double k[4];
-        stage_block<P, T>(W, args.mat, el, a, gN, k);
-        double *dst = acc + 4 * (lbase[ownA] + int(rk[a * N]));
-        if (DET) {
-#pragma unroll
-          for (int c = 0; c < 4; c++) dst[c] += k[c];
-        } else {
-#pragma unroll
-          for (int c = 0; c < 4; c++) atomicAdd(dst + c, k[c]);
-        }
-      }
     }
   }
   wave_sync();
@@ -366,10 +421,12 @@ __global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args)
   using O = Ord<P>;
   __shared__ WaveLDS<P, T> wl[WAVES];
   __shared__ Basis1D basis;
+  __shared__ double2 gref[O::Q * O::N];
   __shared__ __align__(16) double acc[JAC ? 4 * MAXB : 2];
   __shared__ __align__(16) double racc[RES ? 2 * MAXR : 2];
   __shared__ int lrows[MAXR];
   __shared__ int lbase[MAXR];
+  __shared__ int4 lruns[JAC ? MAXR : 1];
 
   // XCD-aware order: consecutive workgroup ids land on different XCDs, so give each XCD a
   // contiguous range of (spatially adjacent) parts to keep shared element data in its L2.
@@ -378,53 +435,79 @@ __global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args)
   const int part = int(blockIdx.x % nxcd) * per + int(blockIdx.x / nxcd);
   if (part >= args.numParts) return;
 
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int el = lane / O::N, idx = lane % O::N;
+  const bool laneOk = lane < O::LANES;
+  const int e0 = args.partElemPtr[part];
+  const int ne = args.partElemPtr[part + 1] - e0;
+  const int vlast = e0 + ne - 1;  // every part has at least one visit
+  constexpr int STEP = WAVES * O::EPW;
+
+  // Start the first gathers before the part's LDS set-up so their latency overlaps it. Loads are
+  // unconditional (indices clamped into the part) so that no branch sits between a load and its use.
+  Gather<P> cur, nxt;
+  int vcur = wave * O::EPW + el;
+  if (!DET) {
+    gather_level1<P>(args, min(e0 + vcur, vlast), idx, cur);
+    gather_level2<P>(args, idx, cur);
+    gather_level1<P>(args, min(e0 + vcur + STEP, vlast), idx, nxt);
+  }
+
   const int r0 = args.partRowPtr[part];
   const int nr = args.partRowPtr[part + 1] - r0;
   const int nb = args.partBlocks[part];
+  const int rb = args.partRunPtr[part];
+  const int nruns = args.partRunPtr[part + 1] - rb;
   for (int i = threadIdx.x; i < nr; i += blockDim.x) {
     lrows[i] = args.rows[r0 + i];
     lbase[i] = args.rowLbase[r0 + i];
   }
   if (JAC) {
+    for (int i = threadIdx.x; i < nruns; i += blockDim.x) lruns[i] = args.runs[rb + i];
     for (int i = threadIdx.x; i < 2 * nb; i += blockDim.x) reinterpret_cast<double2 *>(acc)[i] = make_double2(0, 0);
   }
   if (RES) {
     for (int i = threadIdx.x; i < nr; i += blockDim.x) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
   }
   copy_basis(basis, args.basis);
+  fill_gref<P>(gref, args.basis);
   __syncthreads();
 
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int el = lane / O::N, idx = lane % O::N;
   WaveLDS<P, T> &W = wl[wave];
-  const int e0 = args.partElemPtr[part];
-  const int ne = args.partElemPtr[part + 1] - e0;
   if (DET) {
     const int *off = args.colourOff + int64_t(part) * (args.numColours + 1);
     for (int c = 0; c < args.numColours; c++) {
       const int cb = off[c], ce = off[c + 1];
-      for (int b = cb + wave * O::EPW; b < ce; b += WAVES * O::EPW) {
-        owner_batch<P, T, WAVES, MAXB, MAXR, RES, JAC, DET>(args, W, basis, acc, racc, lrows, lbase, nr, e0, b, ce, el,
-                                                            idx, lane);
+      for (int b = cb + wave * O::EPW; b < ce; b += STEP) {
+        const bool ok = laneOk && b + el < ce;
+        Gather<P> g;
+        gather_level1<P>(args, min(e0 + b + el, vlast), idx, g);
+        gather_level2<P>(args, idx, g);
+        owner_batch<P, T, RES, JAC, DET>(args, W, basis, gref, acc, racc, lrows, lbase, nr, g, ok, el, idx);
       }
       __syncthreads();
     }
   } else {
-    for (int b = wave * O::EPW; b < ne; b += WAVES * O::EPW) {
-      owner_batch<P, T, WAVES, MAXB, MAXR, RES, JAC, DET>(args, W, basis, acc, racc, lrows, lbase, nr, e0, b, ne, el,
-                                                          idx, lane);
+    // Software pipeline: gathers for visit batch k+1 (coordinates, states, ranks) and k+2 (node ids)
+    // are in flight while batch k is computed.
+    for (; vcur - el < ne; vcur += STEP) {
+      const bool ok = laneOk && vcur < ne;
+      const Gather<P> g = cur;
+      cur = nxt;
+      gather_level2<P>(args, idx, cur);
+      gather_level1<P>(args, min(e0 + vcur + 2 * STEP, vlast), idx, nxt);
+      owner_batch<P, T, RES, JAC, DET>(args, W, basis, gref, acc, racc, lrows, lbase, nr, g, ok, el, idx);
     }
     __syncthreads();
   }
 
-  // Write-once output: BSR runs are contiguous in LDS and in the value array.
+  // Write-once output: each BSR run is contiguous in LDS and in the value array; one wave per run.
   if (JAC) {
-    const int rb = args.partRunPtr[part], re = args.partRunPtr[part + 1];
-    for (int r = rb; r < re; r++) {
-      const int4 run = args.runs[r];
+    for (int r = wave; r < nruns; r += WAVES) {
+      const int4 run = lruns[r];
       const double2 *src = reinterpret_cast<const double2 *>(acc) + 2 * run.x;
       double2 *dst = reinterpret_cast<double2 *>(args.values) + 2 * int64_t(run.y);
-      for (int i = threadIdx.x; i < 2 * run.z; i += blockDim.x) dst[i] = src[i];
+      for (int i = lane; i < 2 * run.z; i += 64) dst[i] = src[i];
     }
   }
   if (RES) {
@@ -435,20 +518,25 @@ __global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args)
 }
 
 // ---- BSR SpMV ---------------------------------------------------------------------------------------
-__global__ void bsr_spmv_kernel(int numRows, const int *rowp, const int *cols, const double *values, const double *x,
-                                double *y) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per block row; lanes stride over the row's blocks and reduce.
+__global__ __launch_bounds__(256) void bsr_spmv_kernel(int numRows, const int *rowp, const int *cols,
+                                                       const double *values, const double *x, double *y) {
+  const int r = blockIdx.x * 4 + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
   if (r >= numRows) return;
   double y0 = 0, y1 = 0;
-  for (int k = rowp[r]; k < rowp[r + 1]; k++) {
-    const double *A = values + 4 * int64_t(k);
-    const int c = cols[k];
-    const double x0 = x[2 * int64_t(c)], x1 = x[2 * int64_t(c) + 1];
-    y0 += A[0] * x0 + A[1] * x1;
-    y1 += A[2] * x0 + A[3] * x1;
+  for (int k = rowp[r] + lane; k < rowp[r + 1]; k += 64) {
+    const double4 A = *reinterpret_cast<const double4 *>(values + 4 * int64_t(k));
+    const double2 xv = *reinterpret_cast<const double2 *>(x + 2 * int64_t(cols[k]));
+    y0 += A.x * xv.x + A.y * xv.y;
+    y1 += A.z * xv.x + A.w * xv.y;
   }
-  y[2 * int64_t(r)] = y0;
-  y[2 * int64_t(r) + 1] = y1;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    y0 += __shfl_xor(y0, off);
+    y1 += __shfl_xor(y1, off);
+  }
+  if (lane == 0) *reinterpret_cast<double2 *>(y + 2 * int64_t(r)) = make_double2(y0, y1);
 }
 
 // ---- host helpers -------------------------------------------------------------------------------------
@@ -539,27 +627,25 @@ void launch_element_order(int order, const ElemArgs &a, int precision, hipStream
   }
 }
 
-template <int P, typename T, bool RES, bool JAC, bool DET>
+template <int P, typename T, int MAXR, bool RES, bool JAC, bool DET>
 void launch_owner(const OwnerArgs &a, hipStream_t stream) {
   constexpr OwnerConfig cfg = owner_config(P);
-  constexpr int MAXR = (JAC || !RES) ? cfg.maxRows : kResidualOnlyMaxRows;
   const int grid = ((a.numParts + 7) / 8) * 8;
   owner_kernel<P, T, cfg.waves, cfg.maxBlocks, MAXR, RES, JAC, DET><<<grid, cfg.waves * 64, 0, stream>>>(a);
 }
 
 template <int P, typename T, bool DET>
 void launch_owner_mode(const OwnerArgs &a, bool res, bool jac, bool residualOnlyParts, hipStream_t stream) {
+  constexpr OwnerConfig cfg = owner_config(P);
   if (jac && res) {
-    launch_owner<P, T, true, true, DET>(a, stream);
+    launch_owner<P, T, cfg.maxRows, true, true, DET>(a, stream);
   } else if (jac) {
-    launch_owner<P, T, false, true, DET>(a, stream);
+    launch_owner<P, T, cfg.maxRows, false, true, DET>(a, stream);
   } else if (residualOnlyParts) {
-    launch_owner<P, T, true, false, DET>(a, stream);
+    launch_owner<P, T, kResidualOnlyMaxRows, true, false, DET>(a, stream);
   } else {
     // Residual from a plan whose parts were sized for the Jacobian: rows fit the smaller bound.
-    constexpr OwnerConfig cfg = owner_config(P);
-    const int grid = ((a.numParts + 7) / 8) * 8;
-    owner_kernel<P, T, cfg.waves, cfg.maxBlocks, cfg.maxRows, true, false, DET><<<grid, cfg.waves * 64, 0, stream>>>(a);
+    launch_owner<P, T, cfg.maxRows, true, false, DET>(a, stream);
   }
 }
 
@@ -598,8 +684,9 @@ int upload(const std::vector<TPtr> &h, TPtr **d, int64_t &bytes) {
 }
 
 void free_plan_arrays(fem_plan *p) {
-  void *ptrs[] = {p->dConn,    p->dNodeLrow,    p->dPartRowPtr, p->dRows,  p->dRowLbase, p->dPartBlocks,
-                  p->dPartRunPtr, p->dRuns,     p->dPartElemPtr, p->dElems, p->dColourOff, p->dRank};
+  void *ptrs[] = {p->dVisitConn,  p->dVisitEid, p->dNodeLrow,    p->dPartRowPtr, p->dRows,
+                  p->dRowLbase,   p->dPartBlocks, p->dPartRunPtr, p->dRuns,       p->dPartElemPtr,
+                  p->dColourOff,  p->dRank};
   for (void *q : ptrs) {
     if (q) (void)hipFree(q);
   }
@@ -718,14 +805,14 @@ int fem_plan_create(int order, int numNodes, int numElements, const int *connPtr
   p->numColours = h.numColours;
   p->flags = flags;
   p->numBlocks = h.numBlocks;
-  p->elementVisits = int64_t(h.elems.size());
+  p->elementVisits = int64_t(h.visitEid.size());
   (void)hipGetDevice(&p->device);
   int64_t bytes = 0;
-  if ((rc = upload(h.conn, &p->dConn, bytes)) || (rc = upload(h.nodeLrow, &p->dNodeLrow, bytes)) ||
-      (rc = upload(h.partRowPtr, &p->dPartRowPtr, bytes)) || (rc = upload(h.rows, &p->dRows, bytes)) ||
-      (rc = upload(h.rowLbase, &p->dRowLbase, bytes)) || (rc = upload(h.partBlocks, &p->dPartBlocks, bytes)) ||
-      (rc = upload(h.partRunPtr, &p->dPartRunPtr, bytes)) || (rc = upload(h.runs, &p->dRuns, bytes)) ||
-      (rc = upload(h.partElemPtr, &p->dPartElemPtr, bytes)) || (rc = upload(h.elems, &p->dElems, bytes)) ||
+  if ((rc = upload(h.visitConn, &p->dVisitConn, bytes)) || (rc = upload(h.visitEid, &p->dVisitEid, bytes)) ||
+      (rc = upload(h.nodeLrow, &p->dNodeLrow, bytes)) || (rc = upload(h.partRowPtr, &p->dPartRowPtr, bytes)) ||
+      (rc = upload(h.rows, &p->dRows, bytes)) || (rc = upload(h.rowLbase, &p->dRowLbase, bytes)) ||
+      (rc = upload(h.partBlocks, &p->dPartBlocks, bytes)) || (rc = upload(h.partRunPtr, &p->dPartRunPtr, bytes)) ||
+      (rc = upload(h.runs, &p->dRuns, bytes)) || (rc = upload(h.partElemPtr, &p->dPartElemPtr, bytes)) ||
       (rc = upload(h.colourOff, &p->dColourOff, bytes)) || (rc = upload(h.rank, &p->dRank, bytes))) {
     free_plan_arrays(p);
     delete p;
@@ -759,7 +846,8 @@ int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCo
     return set_error(FEM_ERR_INVALID_ARGUMENT, "plan was built residual-only; it cannot assemble the Jacobian");
   }
   OwnerArgs a{};
-  a.conn = plan->dConn;
+  a.visitConn = plan->dVisitConn;
+  a.visitEid = plan->dVisitEid;
   a.nodeLrow = plan->dNodeLrow;
   a.partRowPtr = plan->dPartRowPtr;
   a.rows = plan->dRows;
@@ -768,9 +856,8 @@ int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCo
   a.partRunPtr = plan->dPartRunPtr;
   a.runs = reinterpret_cast<const int4 *>(plan->dRuns);
   a.partElemPtr = plan->dPartElemPtr;
-  a.elems = plan->dElems;
   a.colourOff = plan->dColourOff;
-  a.rank = plan->dRank;
+  a.rank = reinterpret_cast<const uint32_t *>(plan->dRank);
   a.numParts = plan->numParts;
   a.numColours = plan->numColours;
   a.states = states;
@@ -805,7 +892,7 @@ int fem_bsr_spmv(int numRows, const int *rowp, const int *cols, const double *va
   if (numRows == 0) return FEM_OK;
   if (!rowp || !cols || !values || !x || !y) return set_error(FEM_ERR_NULL_POINTER, "NULL pointer");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  bsr_spmv_kernel<<<(numRows + 255) / 256, 256, 0, s>>>(numRows, rowp, cols, values, x, y);
+  bsr_spmv_kernel<<<(numRows + 3) / 4, 256, 0, s>>>(numRows, rowp, cols, values, x, y);
   FEM_HIP_CHECK(hipGetLastError());
   return FEM_OK;
 }

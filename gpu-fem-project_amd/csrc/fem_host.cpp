@@ -203,16 +203,14 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
                      out.maxRowLength, maxBlocks);
   }
 
-  // Dense element-major connectivity.
-  out.conn.resize(size_t(numElements) * N);
-  for (int e = 0; e < numElements; e++) std::memcpy(&out.conn[size_t(e) * N], conn + connPtr[e], N * sizeof(int));
-
-  // Column ranks of every element block, checked against the pattern.
-  out.rank.resize(size_t(numElements) * N * N);
+  // Column ranks of every element block, checked against the pattern, stored transposed and padded:
+  // rank[(e*N + j)*RP + i] = slot of column conn[e,j] within row conn[e,i].
+  const int RP = rank_pad(order);
+  out.rank.assign(size_t(numElements) * N * RP, 0);
   std::atomic<int> missE{-1};
   parallel_for(numElements, [&](int64_t b, int64_t e) {
     for (int64_t el = b; el < e; el++) {
-      const int *en = &out.conn[size_t(el) * N];
+      const int *en = conn + connPtr[el];
       for (int i = 0; i < N; i++) {
         const int r = en[i];
         for (int j = 0; j < N; j++) {
@@ -221,7 +219,7 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
             missE.store(int(el));
             return;
           }
-          out.rank[(size_t(el) * N + i) * N + j] = uint8_t(pos - rowp[r]);
+          out.rank[(size_t(el) * N + j) * RP + i] = uint8_t(pos - rowp[r]);
         }
       }
     }
@@ -339,8 +337,15 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
     for (int el : partElems[p]) off[colour[el] + 1]++;
     for (int c = 0; c < numColours; c++) off[c + 1] += off[c];
   }
-  out.elems.reserve(size_t(out.partElemPtr[P]));
-  for (int p = 0; p < P; p++) out.elems.insert(out.elems.end(), partElems[p].begin(), partElems[p].end());
+  const size_t visits = size_t(out.partElemPtr[P]);
+  out.visitEid.reserve(visits);
+  for (int p = 0; p < P; p++) out.visitEid.insert(out.visitEid.end(), partElems[p].begin(), partElems[p].end());
+  out.visitConn.resize(visits * N);
+  parallel_for(int64_t(visits), [&](int64_t b, int64_t e) {
+    for (int64_t v = b; v < e; v++) {
+      std::memcpy(&out.visitConn[size_t(v) * N], conn + connPtr[out.visitEid[size_t(v)]], N * sizeof(int));
+    }
+  });
   return FEM_OK;
 }
 

@@ -12,7 +12,9 @@ namespace fem {
 int set_error(int code, const char *fmt, ...);
 void clear_error();
 
-inline int nodes_per_element(int order) { return (order + 1) * (order + 1); }
+constexpr int nodes_per_element(int order) { return (order + 1) * (order + 1); }
+// Bytes of column rank stored per (element, column node): N rounded up to whole 32-bit words.
+constexpr int rank_pad(int order) { return (nodes_per_element(order) + 3) / 4 * 4; }
 
 // Launch geometry of the owner-computes kernel, shared by the host partitioner (which sizes parts
 // to fit) and the kernel templates (which size their LDS from it).
@@ -24,11 +26,13 @@ struct OwnerConfig {
   int maxBlocks;
   int maxRows;
 };
+// Sized so that two workgroups fit the 160 KB LDS of a CU (per-wave staging of 7-9.5 KB, the FP64
+// block accumulator and the row tables).
 constexpr OwnerConfig owner_config(int order) {
-  return order == 1   ? OwnerConfig{4, 2048, 512}
-         : order == 2 ? OwnerConfig{4, 1536, 256}
-         : order == 3 ? OwnerConfig{2, 1024, 128}
-                      : OwnerConfig{2, 1024, 64};
+  return order == 1   ? OwnerConfig{4, 1088, 160}
+         : order == 2 ? OwnerConfig{4, 1152, 160}
+         : order == 3 ? OwnerConfig{4, 1120, 96}
+                      : OwnerConfig{4, 1216, 64};
 }
 // Residual-only parts: only the per-row residual lives in LDS.
 constexpr int kResidualOnlyMaxRows = 1024;
@@ -42,7 +46,8 @@ struct PlanHost {
   int numColours = 1;
   int maxRowLength = 0;
   int64_t numBlocks = 0;
-  std::vector<int> conn;          // dense element-major [E*N]
+  std::vector<int> visitConn;     // [visits*N] nodes of each element visit (parts' visits contiguous)
+  std::vector<int> visitEid;      // [visits] element id of each visit
   std::vector<int> nodeLrow;      // [numNodes] index of the node within its part's row list
   std::vector<int> partRowPtr;    // [P+1]
   std::vector<int> rows;          // global row (node) ids, ascending within a part
@@ -50,10 +55,9 @@ struct PlanHost {
   std::vector<int> partBlocks;    // [P] blocks owned by the part
   std::vector<int> partRunPtr;    // [P+1]
   std::vector<int> runs;          // [R*4] (lds block start, global block start, blocks, 0)
-  std::vector<int> partElemPtr;   // [P+1]
-  std::vector<int> elems;         // element ids touching the part, grouped by colour
-  std::vector<int> colourOff;     // [P*(C+1)] offsets of each colour within a part's element list
-  std::vector<uint8_t> rank;      // [E*N*N] position of block (conn[e,i], conn[e,j]) in row conn[e,i]
+  std::vector<int> partElemPtr;   // [P+1] visit range of each part; visits grouped by colour
+  std::vector<int> colourOff;     // [P*(C+1)] offsets of each colour within a part's visit list
+  std::vector<uint8_t> rank;      // [E][N][rank_pad] rank[e][j][i]: slot of column conn[e,j] in row conn[e,i]
 };
 
 int build_plan_host(int order, int numNodes, int numElements, const int *connPtr, const int *conn,
@@ -73,7 +77,8 @@ struct fem_plan {
   int64_t numBlocks = 0;
   int64_t elementVisits = 0;
   int64_t deviceBytes = 0;
-  int *dConn = nullptr;
+  int *dVisitConn = nullptr;
+  int *dVisitEid = nullptr;
   int *dNodeLrow = nullptr;
   int *dPartRowPtr = nullptr;
   int *dRows = nullptr;
@@ -82,7 +87,6 @@ struct fem_plan {
   int *dPartRunPtr = nullptr;
   int *dRuns = nullptr;
   int *dPartElemPtr = nullptr;
-  int *dElems = nullptr;
   int *dColourOff = nullptr;
   uint8_t *dRank = nullptr;
 };
