@@ -19,6 +19,7 @@
 // elements of one wavefront fall in distinct banks.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -50,9 +51,9 @@ struct Ord {
   static constexpr int Q = N;
   static constexpr int EPW = 64 / N;
   static constexpr int LANES = EPW * N;
-  static constexpr int RPAD = rank_pad(P);                  // rank bytes stored per (element, column node)
-  static constexpr int AC = P <= 3 ? N : 13;  // rows per stiffness chunk
-  static constexpr int NCHUNK = (N + AC - 1) / AC;
+  static constexpr int RPAD = rank_pad(P);  // rank bytes stored per (element, column node)
+  static constexpr int H = N / 2;
+  static constexpr int D = H + 1;  // stiffness blocks per lane: rows a = b, b+1, ..., b+H (mod N)
   static constexpr int NODAL_STRIDE = 2 * odd_units(2 * N, 16);  // doubles; node a at [4a, 4a+4)
   static constexpr int GEO_STRIDE = 2 * odd_units(4 * Q, 16);    // doubles; point q at [8q, 8q+8)
 };
@@ -79,17 +80,27 @@ struct Vec2<float> {
   using type = float2;
 };
 
-// Per-wave LDS staging.
+// Per-wave LDS staging. The gathered nodes are dead once the geometry is formed, so they share
+// storage with the weighted gradients of the stiffness stage.
 template <int P, typename T>
 struct __align__(16) WaveLDS {
   using O = Ord<P>;
   using V2 = typename Vec2<T>::type;
   static constexpr int SG_STRIDE = odd_units(O::N, int(sizeof(V2)));
-  double nodal[O::EPW][O::NODAL_STRIDE];  // x, y, u, v of each element node
-  double geo[O::EPW][O::GEO_STRIDE];      // per q: J^-1 (row-major), s_q, s_q*sigma_xx, s_q*sigma_yy, s_q*sigma_xy
-  V2 sg[2][O::EPW][SG_STRIDE];            // s_q * gradN_b(q) of the current quadrature point (double-buffered)
-  int own[O::EPW][O::N];                  // owner path: local row of node a in this part, or -1
+  union {
+    double nodal[O::EPW][O::NODAL_STRIDE];  // x, y, u, v of each element node
+    V2 sg[2][O::EPW][SG_STRIDE];            // s_q * gradN_b(q) of the current quadrature point (double-buffered)
+  };
+  double geo[O::EPW][O::GEO_STRIDE];  // per q: J^-1 (row-major), s_q, s_q*sigma_xx, s_q*sigma_yy, s_q*sigma_xy
+  int own[O::EPW][O::N];              // owner path: local row of node a in this part, or -1
+  uint32_t rk[O::EPW][O::N][O::RPAD / 4];  // owner path: rk[el][b] byte a = slot of column b within row a
 };
+
+// Column rank of node `col` within BSR row of node `row` for element `el` (owner path).
+template <int P, typename T>
+__device__ __forceinline__ int rank_of(const WaveLDS<P, T> &W, int el, int row, int col) {
+  return int(reinterpret_cast<const uint8_t *>(&W.rk[el][col][0])[row]);
+}
 
 __device__ __forceinline__ void wave_sync() { __syncwarp(); }
 
@@ -174,84 +185,85 @@ __device__ __forceinline__ void stage_geometry(WaveLDS<P, T> &W, const Basis1D &
   *reinterpret_cast<double2 *>(g + 6) = make_double2(double(ts * syy), double(ts * sxy));
 }
 
-// Residual entries of node b and (JAC) the stiffness blocks K(a, b) of every row a, for lane (el, b).
-// emit(a, k) receives each block, row-major [xx, xy, yx, yy], in FP64. Every lane of the wavefront
-// must call this (it synchronises the wavefront); `valid` masks the lane's LDS writes and emits.
-// The geometry of quadrature point q+1 is read while the Gram sums of q are accumulated.
+// Residual entries of node b and (JAC) stiffness blocks for lane (el, b). K is symmetric, so each
+// unordered node pair {a, b} is formed once: lane b takes rows a = b + d (mod N), d = 0..N/2 (for even
+// N the pair at distance N/2 is taken by the lower lane only). emit(row, col, k) receives each block
+// K(row, col), row-major [xx, xy, yx, yy], in FP64; blocks with a != b are emitted for both (a, b)
+// and (b, a) = K(a, b)^T. Every lane of the wavefront must call this (it synchronises the wavefront);
+// `valid` masks the lane's LDS writes and emits. The geometry of quadrature point q+1 is read while
+// the Gram sums of q are accumulated.
 template <int P, typename T, bool JAC, typename Emit>
 __device__ __forceinline__ void stage_stiffness(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el,
                                                 int b, bool valid, T &Rx, T &Ry, Emit &&emit) {
   using O = Ord<P>;
-  constexpr int AC = O::AC, N = O::N, Q = O::Q;
+  constexpr int N = O::N, Q = O::Q, H = O::H, D = O::D;
   using V2 = typename Vec2<T>::type;
   const T mu = T(m.mu), lam = T(m.lambda), c1 = lam + 2 * mu;
   Rx = 0;
   Ry = 0;
   int buf = 0;
+  T S[D][4];
 #pragma unroll
-  for (int c = 0; c < (JAC ? O::NCHUNK : 1); c++) {
-    const int a0 = c * AC;
-    T S[AC][4];
+  for (int i = 0; i < D; i++) S[i][0] = S[i][1] = S[i][2] = S[i][3] = 0;
+  // rows handled by this lane
+  int arow[D];
 #pragma unroll
-    for (int i = 0; i < AC; i++) S[i][0] = S[i][1] = S[i][2] = S[i][3] = 0;
-    // quadrature point 0
-    double2 gg = gref[b];
-    double2 ia = *reinterpret_cast<const double2 *>(&W.geo[el][0]);
-    double2 ib = *reinterpret_cast<const double2 *>(&W.geo[el][2]);
-    double2 sc = *reinterpret_cast<const double2 *>(&W.geo[el][4]);
-    double2 sd = *reinterpret_cast<const double2 *>(&W.geo[el][6]);
+  for (int d = 0; d < D; d++) arow[d] = b + d < N ? b + d : b + d - N;
+  const bool lastPair = (N % 2 == 1) || b < H;  // even N: distance-N/2 pair taken by the lower lane
+  double2 gg = gref[b];
+  double2 ia = *reinterpret_cast<const double2 *>(&W.geo[el][0]);
+  double2 ib = *reinterpret_cast<const double2 *>(&W.geo[el][2]);
+  double2 sc = *reinterpret_cast<const double2 *>(&W.geo[el][4]);
+  double2 sd = *reinterpret_cast<const double2 *>(&W.geo[el][6]);
 #pragma unroll 1
-    for (int q = 0; q < Q; q++) {
-      const T gx = T(gg.x), gy = T(gg.y);
-      const T nx = gx * T(ia.x) + gy * T(ib.x);
-      const T ny = gx * T(ia.y) + gy * T(ib.y);
-      if (c == 0) {
-        const T txx = T(sc.y), tyy = T(sd.x), txy = T(sd.y);
-        Rx += txx * nx + txy * ny;
-        Ry += txy * nx + tyy * ny;
-      }
-      V2 v;
-      if (JAC) {
-        const T s = T(sc.x);
-        v.x = s * nx;
-        v.y = s * ny;
-      }
-      // next quadrature point's geometry (clamped: the last iteration re-reads point Q-1)
-      const int qn = q + 1 < Q ? q + 1 : q;
-      const double *g = &W.geo[el][8 * qn];
-      gg = gref[qn * N + b];
-      ia = *reinterpret_cast<const double2 *>(g + 0);
-      ib = *reinterpret_cast<const double2 *>(g + 2);
-      sc = *reinterpret_cast<const double2 *>(g + 4);
-      if (c == 0) sd = *reinterpret_cast<const double2 *>(g + 6);
-      if (JAC) {
-        if (valid) W.sg[buf][el][b] = v;
-        wave_sync();
-        const V2 *row = &W.sg[buf][el][a0];
-#pragma unroll
-        for (int i = 0; i < AC; i++) {
-          if (a0 + i < N) {
-            const V2 w = row[i];
-            S[i][0] = fma(w.x, nx, S[i][0]);
-            S[i][1] = fma(w.y, ny, S[i][1]);
-            S[i][2] = fma(w.x, ny, S[i][2]);
-            S[i][3] = fma(w.y, nx, S[i][3]);
-          }
-        }
-        buf ^= 1;
-      }
-    }
+  for (int q = 0; q < Q; q++) {
+    const T gx = T(gg.x), gy = T(gg.y);
+    const T nx = gx * T(ia.x) + gy * T(ib.x);
+    const T ny = gx * T(ia.y) + gy * T(ib.y);
+    const T txx = T(sc.y), tyy = T(sd.x), txy = T(sd.y);
+    Rx += txx * nx + txy * ny;
+    Ry += txy * nx + tyy * ny;
+    V2 v;
     if (JAC) {
+      const T s = T(sc.x);
+      v.x = s * nx;
+      v.y = s * ny;
+    }
+    // next quadrature point's geometry (clamped: the last iteration re-reads point Q-1)
+    const int qn = q + 1 < Q ? q + 1 : q;
+    const double *g = &W.geo[el][8 * qn];
+    gg = gref[qn * N + b];
+    ia = *reinterpret_cast<const double2 *>(g + 0);
+    ib = *reinterpret_cast<const double2 *>(g + 2);
+    sc = *reinterpret_cast<const double2 *>(g + 4);
+    sd = *reinterpret_cast<const double2 *>(g + 6);
+    if (JAC) {
+      if (valid) W.sg[buf][el][b] = v;
+      wave_sync();
 #pragma unroll
-      for (int i = 0; i < AC; i++) {
-        if (a0 + i < N) {
-          double k[4];
-          k[0] = double(c1 * S[i][0] + mu * S[i][1]);
-          k[1] = double(lam * S[i][2] + mu * S[i][3]);
-          k[2] = double(lam * S[i][3] + mu * S[i][2]);
-          k[3] = double(c1 * S[i][1] + mu * S[i][0]);
-          if (valid) emit(a0 + i, k);
-        }
+      for (int d = 0; d < D; d++) {
+        const V2 w = W.sg[buf][el][arow[d]];
+        S[d][0] = fma(w.x, nx, S[d][0]);
+        S[d][1] = fma(w.y, ny, S[d][1]);
+        S[d][2] = fma(w.x, ny, S[d][2]);
+        S[d][3] = fma(w.y, nx, S[d][3]);
+      }
+      buf ^= 1;
+    }
+  }
+  if (JAC && valid) {
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      if (d == H && !lastPair) continue;
+      double k[4];
+      k[0] = double(c1 * S[d][0] + mu * S[d][1]);
+      k[1] = double(lam * S[d][2] + mu * S[d][3]);
+      k[2] = double(lam * S[d][3] + mu * S[d][2]);
+      k[3] = double(c1 * S[d][1] + mu * S[d][0]);
+      emit(arow[d], b, k);
+      if (d != 0) {
+        const double kt[4] = {k[0], k[2], k[1], k[3]};
+        emit(b, arow[d], kt);
       }
     }
   }
@@ -299,10 +311,10 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   wave_sync();
   if (valid) stage_geometry<P, T>(W, basis, args.mat, el, idx);
   wave_sync();
-  const int *mrow = args.map + int64_t(eid) * N * N + idx;
+  const int *mel = args.map + int64_t(eid) * N * N;
   T Rx, Ry;
-  stage_stiffness<P, T, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, [&](int a, const double(&k)[4]) {
-    double *blk = args.values + mrow[a * N];
+  stage_stiffness<P, T, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, [&](int row, int col, const double(&k)[4]) __attribute__((always_inline)) {
+    double *blk = args.values + mel[row * N + col];
     if (ATOMIC) {
 #pragma unroll
       for (int c = 0; c < 4; c++) unsafeAtomicAdd(blk + c, k[c]);
@@ -337,6 +349,7 @@ struct OwnerArgs {
   const int *partElemPtr;
   const int *colourOff;
   const uint32_t *rank;  // [E][N][RPAD] bytes: rank[e][b][a] = slot of column conn[e,b] within row conn[e,a]
+  const int4 *partDesc;  // [P][2]: (first row, rows, blocks, first run), (runs, first visit, visits, 0)
   int numParts;
   int numColours;
   const double *states;
@@ -386,16 +399,17 @@ __device__ __forceinline__ void owner_batch(const OwnerArgs &args, WaveLDS<P, T>
     own = (g.lrow < nr && lrows[g.lrow] == g.node) ? g.lrow : -1;
     store_nodal(W, el, idx, g.xy, g.uv);
     W.own[el][idx] = own;
+#pragma unroll
+    for (int i = 0; i < Ord<P>::RPAD / 4; i++) W.rk[el][idx][i] = g.rk[i];
   }
   wave_sync();
   if (valid) stage_geometry<P, T>(W, basis, args.mat, el, idx);
   wave_sync();
   T Rx, Ry;
-  stage_stiffness<P, T, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, [&](int a, const double(&k)[4]) {
-    const int ownA = W.own[el][a];
-    if (ownA < 0) return;
-    const int r = int((g.rk[a >> 2] >> (8 * (a & 3))) & 0xFFu);
-    double *dst = acc + 4 * (lbase[ownA] + r);
+  stage_stiffness<P, T, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, [&](int row, int col, const double(&k)[4]) __attribute__((always_inline)) {
+    const int ownR = W.own[el][row];
+    if (ownR < 0) return;
+    double *dst = acc + 4 * (lbase[ownR] + rank_of(W, el, row, col));
     if (DET) {
 #pragma unroll
       for (int c = 0; c < 4; c++) dst[c] += k[c];
@@ -517,6 +531,118 @@ __global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args)
   }
 }
 
+// Streaming variant for plans whose parts need one batch per wavefront (the default partition):
+// a persistent workgroup walks a contiguous range of parts and, while part k is computed, loads
+// part k+1's row tables and element gathers and part k+2's descriptor. Only the accumulator set-up
+// and the write-out separate two parts.
+struct PartTables {
+  int row, lbase;
+  int runLds, runGlobal, runLen;
+};
+
+__device__ __forceinline__ void load_part_desc(const OwnerArgs &a, int part, int4 &d0, int4 &d1) {
+  d0 = a.partDesc[2 * part];
+  d1 = a.partDesc[2 * part + 1];
+}
+
+__device__ __forceinline__ void load_part_tables(const OwnerArgs &a, const int4 &d0, const int4 &d1, int tid,
+                                                 PartTables &t) {
+  const int i = min(tid, d0.y - 1);
+  t.row = a.rows[d0.x + i];
+  t.lbase = a.rowLbase[d0.x + i];
+  const int4 r = a.runs[d0.w + min(tid, d1.x - 1)];
+  t.runLds = r.x;
+  t.runGlobal = r.y;
+  t.runLen = r.z;
+}
+
+template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC>
+__global__ __launch_bounds__(WAVES * 64) void owner_stream_kernel(const OwnerArgs args) {
+  using O = Ord<P>;
+  static_assert(MAXR <= WAVES * 64, "one owned row per thread");
+  __shared__ WaveLDS<P, T> wl[WAVES];
+  __shared__ Basis1D basis;
+  __shared__ double2 gref[O::Q * O::N];
+  __shared__ __align__(16) double acc[JAC ? 4 * MAXB : 2];
+  __shared__ __align__(16) double racc[RES ? 2 * MAXR : 2];
+  __shared__ int lrows[MAXR];
+  __shared__ int lbase[MAXR];
+  __shared__ int4 lruns[JAC ? MAXR : 1];
+
+  // Contiguous part range per workgroup; workgroup w runs on XCD w % 8, and each XCD's workgroups
+  // share one contiguous (spatially adjacent, Morton-ordered) stretch of parts.
+  const int xcd = int(blockIdx.x % 8), slot = int(blockIdx.x / 8), perX = int(gridDim.x / 8);
+  const int64_t np = args.numParts;
+  const int xb = int(np * xcd / 8), xe = int(np * (xcd + 1) / 8);
+  const int pb = xb + int(int64_t(xe - xb) * slot / perX);
+  const int pe = xb + int(int64_t(xe - xb) * (slot + 1) / perX);
+  if (pb >= pe) return;
+
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int el = lane / O::N, idx = lane % O::N;
+  const bool laneOk = lane < O::LANES;
+  const int myv = wave * O::EPW + el;  // this lane's element visit within a part
+  WaveLDS<P, T> &W = wl[wave];
+
+  int4 c0, c1, n0, n1, nn0, nn1;
+  PartTables tc, tn;
+  Gather<P> gc, gn;
+  load_part_desc(args, pb, c0, c1);
+  load_part_tables(args, c0, c1, tid, tc);
+  gather_level1<P>(args, c1.y + min(myv, c1.z - 1), idx, gc);
+  gather_level2<P>(args, idx, gc);
+  load_part_desc(args, min(pb + 1, pe - 1), n0, n1);
+  copy_basis(basis, args.basis);
+  fill_gref<P>(gref, args.basis);
+
+  for (int part = pb; part < pe; part++) {
+    __syncthreads();  // the previous part's write-out has read the accumulators
+    if (tid < c0.y) {
+      lrows[tid] = tc.row;
+      lbase[tid] = tc.lbase;
+    }
+    if (JAC) {
+      if (tid < c1.x) lruns[tid] = make_int4(tc.runLds, tc.runGlobal, tc.runLen, 0);
+      for (int i = tid; i < 2 * c0.z; i += blockDim.x) reinterpret_cast<double2 *>(acc)[i] = make_double2(0, 0);
+    }
+    if (RES) {
+      for (int i = tid; i < c0.y; i += blockDim.x) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
+    }
+    __syncthreads();
+
+    // Prefetch: part+1 tables and node/element ids, part+2 descriptor (indices clamped to the range).
+    load_part_tables(args, n0, n1, tid, tn);
+    gather_level1<P>(args, n1.y + min(myv, n1.z - 1), idx, gn);
+    load_part_desc(args, min(part + 2, pe - 1), nn0, nn1);
+
+    const bool ok = laneOk && myv < c1.z;
+    owner_batch<P, T, RES, JAC, false>(args, W, basis, gref, acc, racc, lrows, lbase, c0.y, gc, ok, el, idx);
+
+    gather_level2<P>(args, idx, gn);  // part+1 coordinates, states, ranks
+    __syncthreads();
+
+    if (JAC) {
+      for (int r = wave; r < c1.x; r += WAVES) {
+        const int4 run = lruns[r];
+        const double2 *src = reinterpret_cast<const double2 *>(acc) + 2 * run.x;
+        double2 *dst = reinterpret_cast<double2 *>(args.values) + 2 * int64_t(run.y);
+        for (int i = lane; i < 2 * run.z; i += 64) dst[i] = src[i];
+      }
+    }
+    if (RES) {
+      for (int i = tid; i < c0.y; i += blockDim.x) {
+        reinterpret_cast<double2 *>(args.residual)[lrows[i]] = reinterpret_cast<const double2 *>(racc)[i];
+      }
+    }
+    c0 = n0;
+    c1 = n1;
+    n0 = nn0;
+    n1 = nn1;
+    tc = tn;
+    gc = gn;
+  }
+}
+
 // ---- BSR SpMV ---------------------------------------------------------------------------------------
 // One wavefront per block row; lanes stride over the row's blocks and reduce.
 __global__ __launch_bounds__(256) void bsr_spmv_kernel(int numRows, const int *rowp, const int *cols,
@@ -627,9 +753,19 @@ void launch_element_order(int order, const ElemArgs &a, int precision, hipStream
   }
 }
 
+// Streaming (persistent) launches: workgroups resident at once on the device.
+int g_streamGrid = 0;
+
 template <int P, typename T, int MAXR, bool RES, bool JAC, bool DET>
 void launch_owner(const OwnerArgs &a, hipStream_t stream) {
   constexpr OwnerConfig cfg = owner_config(P);
+  if constexpr (!DET && MAXR <= cfg.waves * 64) {
+    if (g_streamGrid > 0) {
+      const int grid = std::min(g_streamGrid, ((a.numParts + 7) / 8) * 8);
+      owner_stream_kernel<P, T, cfg.waves, cfg.maxBlocks, MAXR, RES, JAC><<<grid, cfg.waves * 64, 0, stream>>>(a);
+      return;
+    }
+  }
   const int grid = ((a.numParts + 7) / 8) * 8;
   owner_kernel<P, T, cfg.waves, cfg.maxBlocks, MAXR, RES, JAC, DET><<<grid, cfg.waves * 64, 0, stream>>>(a);
 }
@@ -686,7 +822,7 @@ int upload(const std::vector<TPtr> &h, TPtr **d, int64_t &bytes) {
 void free_plan_arrays(fem_plan *p) {
   void *ptrs[] = {p->dVisitConn,  p->dVisitEid, p->dNodeLrow,    p->dPartRowPtr, p->dRows,
                   p->dRowLbase,   p->dPartBlocks, p->dPartRunPtr, p->dRuns,       p->dPartElemPtr,
-                  p->dColourOff,  p->dRank};
+                  p->dColourOff,  p->dPartDesc,   p->dRank};
   for (void *q : ptrs) {
     if (q) (void)hipFree(q);
   }
@@ -806,14 +942,17 @@ int fem_plan_create(int order, int numNodes, int numElements, const int *connPtr
   p->flags = flags;
   p->numBlocks = h.numBlocks;
   p->elementVisits = int64_t(h.visitEid.size());
+  p->maxPartVisits = h.maxPartVisits;
   (void)hipGetDevice(&p->device);
+  if (hipDeviceGetAttribute(&p->numCUs, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess) p->numCUs = 0;
   int64_t bytes = 0;
   if ((rc = upload(h.visitConn, &p->dVisitConn, bytes)) || (rc = upload(h.visitEid, &p->dVisitEid, bytes)) ||
       (rc = upload(h.nodeLrow, &p->dNodeLrow, bytes)) || (rc = upload(h.partRowPtr, &p->dPartRowPtr, bytes)) ||
       (rc = upload(h.rows, &p->dRows, bytes)) || (rc = upload(h.rowLbase, &p->dRowLbase, bytes)) ||
       (rc = upload(h.partBlocks, &p->dPartBlocks, bytes)) || (rc = upload(h.partRunPtr, &p->dPartRunPtr, bytes)) ||
       (rc = upload(h.runs, &p->dRuns, bytes)) || (rc = upload(h.partElemPtr, &p->dPartElemPtr, bytes)) ||
-      (rc = upload(h.colourOff, &p->dColourOff, bytes)) || (rc = upload(h.rank, &p->dRank, bytes))) {
+      (rc = upload(h.colourOff, &p->dColourOff, bytes)) || (rc = upload(h.partDesc, &p->dPartDesc, bytes)) ||
+      (rc = upload(h.rank, &p->dRank, bytes))) {
     free_plan_arrays(p);
     delete p;
     return rc;
@@ -858,6 +997,7 @@ int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCo
   a.partElemPtr = plan->dPartElemPtr;
   a.colourOff = plan->dColourOff;
   a.rank = reinterpret_cast<const uint32_t *>(plan->dRank);
+  a.partDesc = reinterpret_cast<const int4 *>(plan->dPartDesc);
   a.numParts = plan->numParts;
   a.numColours = plan->numColours;
   a.states = states;
@@ -868,6 +1008,16 @@ int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCo
   if ((rc = make_basis(plan->order, a.basis)) || (rc = make_material(E, nu, t, a.mat))) return rc;
   const bool det = (plan->flags & FEM_PLAN_DETERMINISTIC) != 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  // Persistent streaming kernel when every part is one batch per wavefront (two workgroups per CU).
+  int waves = 4, epw = 64 / nodes_per_element(plan->order);
+  switch (plan->order) {
+    case 1: waves = owner_config(1).waves; break;
+    case 2: waves = owner_config(2).waves; break;
+    case 3: waves = owner_config(3).waves; break;
+    default: waves = owner_config(4).waves; break;
+  }
+  const bool stream1 = !rOnly && plan->numCUs > 0 && plan->maxPartVisits <= waves * epw;
+  g_streamGrid = stream1 ? ((2 * plan->numCUs + 7) / 8) * 8 : 0;
   return timed(s, elapsedSeconds, [&]() {
     switch (plan->order) {
       case 1: launch_owner_order<1>(a, precision, det, residual != nullptr, matEntries != nullptr, rOnly, s); break;

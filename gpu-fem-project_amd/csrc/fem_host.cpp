@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -251,20 +252,48 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
     for (int n = 0; n < numNodes; n++) order_nodes[n] = int(key[n] & 0xFFFFFFFFu);
   }
 
-  // Greedy cut along that order.
+  // Greedy cut along that order. A part closes when its rows would exceed the LDS block budget, the
+  // row table, or the number of element visits one round of the workgroup's wavefronts covers.
+  // FEM_AMD_PART_VISITS / FEM_AMD_PART_BLOCKS lower these bounds (tuning; never above the kernel's).
+  int maxVisits = residualOnly ? (1 << 30) : cfg.maxVisits;
+  int blockBudget = maxBlocks;
+  if (const char *ev = std::getenv("FEM_AMD_PART_VISITS")) {
+    const int v = std::atoi(ev);
+    if (v > 0) maxVisits = v;
+  }
+  if (const char *eb = std::getenv("FEM_AMD_PART_BLOCKS")) {
+    const int v = std::atoi(eb);
+    if (v > 0) blockBudget = std::min(blockBudget, v);
+  }
+  std::vector<int> nptr, nadj;
+  node_element_adjacency(numNodes, numElements, N, connPtr, conn, nptr, nadj);
+  for (int n = 0; n < numNodes; n++) {
+    if (nptr[n + 1] == nptr[n]) {
+      return set_error(FEM_ERR_INVALID_ARGUMENT, "node %d belongs to no element; renumber the mesh without it", n);
+    }
+  }
   out.partRowPtr.push_back(0);
   {
-    int blocks = 0, rowsInPart = 0;
+    std::vector<int> stamp(numElements, -1);
+    int blocks = 0, rowsInPart = 0, visits = 0, part = 0;
     for (int k = 0; k < numNodes; k++) {
       const int n = order_nodes[k];
       const int len = rowp[n + 1] - rowp[n];
-      if (rowsInPart > 0 && (blocks + len > maxBlocks || rowsInPart + 1 > maxRows)) {
+      int newVisits = 0;
+      for (int j = nptr[n]; j < nptr[n + 1]; j++) newVisits += stamp[nadj[size_t(j)]] != part;
+      if (rowsInPart > 0 &&
+          (blocks + len > blockBudget || rowsInPart + 1 > maxRows || visits + newVisits > maxVisits)) {
         out.partRowPtr.push_back(k);
+        part++;
         blocks = 0;
         rowsInPart = 0;
+        visits = 0;
+        newVisits = nptr[n + 1] - nptr[n];
       }
+      for (int j = nptr[n]; j < nptr[n + 1]; j++) stamp[nadj[size_t(j)]] = part;
       blocks += len;
       rowsInPart++;
+      visits += newVisits;
     }
     out.partRowPtr.push_back(numNodes);
   }
@@ -312,8 +341,6 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
   out.numColours = numColours;
 
   // Elements touching each part, grouped by colour then id.
-  std::vector<int> nptr, nadj;
-  node_element_adjacency(numNodes, numElements, N, connPtr, conn, nptr, nadj);
   std::vector<std::vector<int>> partElems(P);
   parallel_for(P, [&](int64_t b, int64_t e) {
     std::vector<int> tmp;
@@ -336,6 +363,18 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
     int *off = &out.colourOff[size_t(p) * (numColours + 1)];
     for (int el : partElems[p]) off[colour[el] + 1]++;
     for (int c = 0; c < numColours; c++) off[c + 1] += off[c];
+  }
+  out.partDesc.assign(size_t(P) * 8, 0);
+  for (int p = 0; p < P; p++) {
+    int *dsc = &out.partDesc[size_t(p) * 8];
+    dsc[0] = out.partRowPtr[p];
+    dsc[1] = out.partRowPtr[p + 1] - out.partRowPtr[p];
+    dsc[2] = out.partBlocks[p];
+    dsc[3] = out.partRunPtr[p];
+    dsc[4] = out.partRunPtr[p + 1] - out.partRunPtr[p];
+    dsc[5] = out.partElemPtr[p];
+    dsc[6] = out.partElemPtr[p + 1] - out.partElemPtr[p];
+    out.maxPartVisits = std::max(out.maxPartVisits, dsc[6]);
   }
   const size_t visits = size_t(out.partElemPtr[P]);
   out.visitEid.reserve(visits);

@@ -21,18 +21,20 @@ constexpr int rank_pad(int order) { return (nodes_per_element(order) + 3) / 4 * 
 //   waves     wavefronts per workgroup
 //   maxBlocks BSR blocks a part may own (FP64 LDS accumulator, 32 B per block)
 //   maxRows   block rows a part may own
+//   maxVisits element visits per part (waves x elements per wavefront: one batch per wave)
 struct OwnerConfig {
   int waves;
   int maxBlocks;
   int maxRows;
+  int maxVisits;
 };
 // Sized so that two workgroups fit the 160 KB LDS of a CU (per-wave staging of 7-9.5 KB, the FP64
 // block accumulator and the row tables).
 constexpr OwnerConfig owner_config(int order) {
-  return order == 1   ? OwnerConfig{4, 1088, 160}
-         : order == 2 ? OwnerConfig{4, 1152, 160}
-         : order == 3 ? OwnerConfig{4, 1120, 96}
-                      : OwnerConfig{4, 1216, 64};
+  return order == 1   ? OwnerConfig{4, 1344, 192, 64}
+         : order == 2 ? OwnerConfig{4, 1344, 160, 28}
+         : order == 3 ? OwnerConfig{4, 1312, 96, 16}
+                      : OwnerConfig{4, 1312, 64, 8};
 }
 // Residual-only parts: only the per-row residual lives in LDS.
 constexpr int kResidualOnlyMaxRows = 1024;
@@ -58,6 +60,8 @@ struct PlanHost {
   std::vector<int> partElemPtr;   // [P+1] visit range of each part; visits grouped by colour
   std::vector<int> colourOff;     // [P*(C+1)] offsets of each colour within a part's visit list
   std::vector<uint8_t> rank;      // [E][N][rank_pad] rank[e][j][i]: slot of column conn[e,j] in row conn[e,i]
+  std::vector<int> partDesc;      // [P*8] (first row, rows, blocks, first run, runs, first visit, visits, 0)
+  int maxPartVisits = 0;
 };
 
 int build_plan_host(int order, int numNodes, int numElements, const int *connPtr, const int *conn,
@@ -88,5 +92,8 @@ struct fem_plan {
   int *dRuns = nullptr;
   int *dPartElemPtr = nullptr;
   int *dColourOff = nullptr;
+  int *dPartDesc = nullptr;
   uint8_t *dRank = nullptr;
+  int maxPartVisits = 0;
+  int numCUs = 0;
 };
