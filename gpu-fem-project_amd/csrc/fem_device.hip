@@ -335,30 +335,44 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   }
 }
 
-// ---- owner-computes kernel (resident plan, write-once) --------------------------------------------
-struct OwnerArgs {
-  const int *visitConn;  // [visits*N] nodes of each element visit, parts' visits contiguous
-  const int *visitEid;   // [visits] element id of each visit
-  const int *nodeLrow;
-  const int *partRowPtr;
-  const int *rows;
-  const int *rowLbase;
-  const int *partBlocks;
-  const int *partRunPtr;
-  const int4 *runs;
-  const int *partElemPtr;
-  const int *colourOff;
-  const uint32_t *rank;  // [E][N][RPAD] bytes: rank[e][b][a] = slot of column conn[e,b] within row conn[e,a]
-  const int4 *partDesc;  // [P][2]: (first row, rows, blocks, first run), (runs, first visit, visits, 0)
-  int numParts;
-  int numColours;
+// ---- owner-computes kernels (resident plans, write-once) -------------------------------------------
+// One assembly of one resident plan: the plan's device arrays and the request's inputs/outputs.
+struct MeshJob {
+  const int *visitConn;   // [visits*N] nodes of each element visit, parts' visits contiguous
+  const int *visitEid;    // [visits] element id of each visit
+  const int *nodeLrow;    // [nodes] index of the node within its part's row list
+  const int *rows;        // [nodes] owned rows, part by part, ascending within a part
+  const int *rowLbase;    // [nodes] LDS block offset of each owned row within its part
+  const int4 *runs;       // [runs] (LDS block start, global block start, blocks, 0)
+  const int4 *partDesc;   // [parts][2] (first row, rows, blocks, first run), (runs, first visit, visits, 0)
+  const uint32_t *rank;   // [E][N][RPAD] bytes: rank[e][b][a] = slot of column conn[e,b] within row conn[e,a]
+  const int *colourOff;   // [parts][C+1] colour offsets within a part's visits (deterministic plans)
   const double *states;
   const double *coords;
-  double *residual;
-  double *values;
-  Basis1D basis;
-  Material mat;
+  double *residual;       // may be NULL
+  double *values;         // may be NULL
+  double mu, lambda, t;
+  int numParts, numColours;
 };
+
+struct OwnerArgs {
+  const MeshJob *jobs;    // device job table
+  const int *jobStart;    // [numJobs+1] first global part of each job
+  int numJobs;
+  int totalParts;
+  Basis1D basis;
+};
+
+struct PartDesc {
+  int r0, nr, nb, rb, nruns, e0, ne;
+};
+
+__device__ __forceinline__ PartDesc part_desc(const MeshJob &J, int p) {
+  const int4 a = J.partDesc[2 * p], b = J.partDesc[2 * p + 1];
+  return PartDesc{a.x, a.y, a.z, a.w, b.x, b.y, b.z};
+}
+
+__device__ __forceinline__ Material job_material(const MeshJob &J) { return Material{J.mu, J.lambda, J.t}; }
 
 // Per-lane gathered inputs of one element visit.
 template <int P>
@@ -372,25 +386,25 @@ struct Gather {
 
 // Level 1: node id and element id of visit `visit` (always in bounds: callers clamp the visit).
 template <int P>
-__device__ __forceinline__ void gather_level1(const OwnerArgs &a, int visit, int idx, Gather<P> &g) {
-  g.node = a.visitConn[int64_t(visit) * Ord<P>::N + idx];
-  g.eid = a.visitEid[visit];
+__device__ __forceinline__ void gather_level1(const MeshJob &J, int visit, int idx, Gather<P> &g) {
+  g.node = J.visitConn[int64_t(visit) * Ord<P>::N + idx];
+  g.eid = J.visitEid[visit];
 }
 
 // Level 2: coordinates, states, owner row and column ranks of the node/element of level 1.
 template <int P>
-__device__ __forceinline__ void gather_level2(const OwnerArgs &a, int idx, Gather<P> &g) {
-  g.xy = *reinterpret_cast<const double2 *>(a.coords + 2 * int64_t(g.node));
-  g.uv = *reinterpret_cast<const double2 *>(a.states + 2 * int64_t(g.node));
-  g.lrow = a.nodeLrow[g.node];
+__device__ __forceinline__ void gather_level2(const MeshJob &J, int idx, Gather<P> &g) {
+  g.xy = *reinterpret_cast<const double2 *>(J.coords + 2 * int64_t(g.node));
+  g.uv = *reinterpret_cast<const double2 *>(J.states + 2 * int64_t(g.node));
+  g.lrow = J.nodeLrow[g.node];
   constexpr int W = Ord<P>::RPAD / 4;
-  const uint32_t *r = a.rank + (int64_t(g.eid) * Ord<P>::N + idx) * W;
+  const uint32_t *r = J.rank + (int64_t(g.eid) * Ord<P>::N + idx) * W;
 #pragma unroll
   for (int i = 0; i < W; i++) g.rk[i] = r[i];
 }
 
 template <int P, typename T, bool RES, bool JAC, bool DET>
-__device__ __forceinline__ void owner_batch(const OwnerArgs &args, WaveLDS<P, T> &W, const Basis1D &basis,
+__device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &W, const Basis1D &basis,
                                             const double2 *gref, double *acc, double *racc, const int *lrows,
                                             const int *lbase, int nr, const Gather<P> &g, bool valid, int el,
                                             int idx) {
@@ -403,10 +417,10 @@ __device__ __forceinline__ void owner_batch(const OwnerArgs &args, WaveLDS<P, T>
     for (int i = 0; i < Ord<P>::RPAD / 4; i++) W.rk[el][idx][i] = g.rk[i];
   }
   wave_sync();
-  if (valid) stage_geometry<P, T>(W, basis, args.mat, el, idx);
+  if (valid) stage_geometry<P, T>(W, basis, mat, el, idx);
   wave_sync();
   T Rx, Ry;
-  stage_stiffness<P, T, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, [&](int row, int col, const double(&k)[4]) __attribute__((always_inline)) {
+  stage_stiffness<P, T, JAC>(W, gref, mat, el, idx, valid, Rx, Ry, [&](int row, int col, const double(&k)[4]) __attribute__((always_inline)) {
     const int ownR = W.own[el][row];
     if (ownR < 0) return;
     double *dst = acc + 4 * (lbase[ownR] + rank_of(W, el, row, col));
@@ -430,6 +444,28 @@ __device__ __forceinline__ void owner_batch(const OwnerArgs &args, WaveLDS<P, T>
   wave_sync();
 }
 
+// Writes a part's accumulated BSR runs (one wave per run) and owned residual rows.
+template <bool RES, bool JAC, int WAVES>
+__device__ __forceinline__ void owner_write(const MeshJob &J, const double *acc, const double *racc,
+                                            const int4 *lruns, int nruns, const int *lrows, int nr) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (JAC && J.values) {
+    for (int r = wave; r < nruns; r += WAVES) {
+      const int4 run = lruns[r];
+      const double2 *src = reinterpret_cast<const double2 *>(acc) + 2 * run.x;
+      double2 *dst = reinterpret_cast<double2 *>(J.values) + 2 * int64_t(run.y);
+      for (int i = lane; i < 2 * run.z; i += 64) dst[i] = src[i];
+    }
+  }
+  if (RES && J.residual) {
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) {
+      reinterpret_cast<double2 *>(J.residual)[lrows[i]] = reinterpret_cast<const double2 *>(racc)[i];
+    }
+  }
+}
+
+// Per-part kernel of a single job (deterministic plans and parts that take several batches per
+// wavefront): one workgroup per part.
 template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC, bool DET>
 __global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args) {
   using O = Ord<P>;
@@ -442,46 +478,31 @@ __global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args)
   __shared__ int lbase[MAXR];
   __shared__ int4 lruns[JAC ? MAXR : 1];
 
+  const MeshJob &J = args.jobs[0];
   // XCD-aware order: consecutive workgroup ids land on different XCDs, so give each XCD a
   // contiguous range of (spatially adjacent) parts to keep shared element data in its L2.
-  const int nxcd = 8;
-  const int per = int(gridDim.x) / nxcd;
-  const int part = int(blockIdx.x % nxcd) * per + int(blockIdx.x / nxcd);
-  if (part >= args.numParts) return;
+  const int per = int(gridDim.x) / 8;
+  const int part = int(blockIdx.x % 8) * per + int(blockIdx.x / 8);
+  if (part >= args.totalParts) return;
 
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int el = lane / O::N, idx = lane % O::N;
   const bool laneOk = lane < O::LANES;
-  const int e0 = args.partElemPtr[part];
-  const int ne = args.partElemPtr[part + 1] - e0;
-  const int vlast = e0 + ne - 1;  // every part has at least one visit
+  const PartDesc d = part_desc(J, part);
+  const int vlast = d.e0 + d.ne - 1;  // every part has at least one visit
   constexpr int STEP = WAVES * O::EPW;
+  const Material mat = job_material(J);
 
-  // Start the first gathers before the part's LDS set-up so their latency overlaps it. Loads are
-  // unconditional (indices clamped into the part) so that no branch sits between a load and its use.
-  Gather<P> cur, nxt;
-  int vcur = wave * O::EPW + el;
-  if (!DET) {
-    gather_level1<P>(args, min(e0 + vcur, vlast), idx, cur);
-    gather_level2<P>(args, idx, cur);
-    gather_level1<P>(args, min(e0 + vcur + STEP, vlast), idx, nxt);
-  }
-
-  const int r0 = args.partRowPtr[part];
-  const int nr = args.partRowPtr[part + 1] - r0;
-  const int nb = args.partBlocks[part];
-  const int rb = args.partRunPtr[part];
-  const int nruns = args.partRunPtr[part + 1] - rb;
-  for (int i = threadIdx.x; i < nr; i += blockDim.x) {
-    lrows[i] = args.rows[r0 + i];
-    lbase[i] = args.rowLbase[r0 + i];
+  for (int i = threadIdx.x; i < d.nr; i += blockDim.x) {
+    lrows[i] = J.rows[d.r0 + i];
+    lbase[i] = J.rowLbase[d.r0 + i];
   }
   if (JAC) {
-    for (int i = threadIdx.x; i < nruns; i += blockDim.x) lruns[i] = args.runs[rb + i];
-    for (int i = threadIdx.x; i < 2 * nb; i += blockDim.x) reinterpret_cast<double2 *>(acc)[i] = make_double2(0, 0);
+    for (int i = threadIdx.x; i < d.nruns; i += blockDim.x) lruns[i] = J.runs[d.rb + i];
+    for (int i = threadIdx.x; i < 2 * d.nb; i += blockDim.x) reinterpret_cast<double2 *>(acc)[i] = make_double2(0, 0);
   }
   if (RES) {
-    for (int i = threadIdx.x; i < nr; i += blockDim.x) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
+    for (int i = threadIdx.x; i < d.nr; i += blockDim.x) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
   }
   copy_basis(basis, args.basis);
   fill_gref<P>(gref, args.basis);
@@ -489,71 +510,62 @@ __global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args)
 
   WaveLDS<P, T> &W = wl[wave];
   if (DET) {
-    const int *off = args.colourOff + int64_t(part) * (args.numColours + 1);
-    for (int c = 0; c < args.numColours; c++) {
+    const int *off = J.colourOff + int64_t(part) * (J.numColours + 1);
+    for (int c = 0; c < J.numColours; c++) {
       const int cb = off[c], ce = off[c + 1];
       for (int b = cb + wave * O::EPW; b < ce; b += STEP) {
         const bool ok = laneOk && b + el < ce;
         Gather<P> g;
-        gather_level1<P>(args, min(e0 + b + el, vlast), idx, g);
-        gather_level2<P>(args, idx, g);
-        owner_batch<P, T, RES, JAC, DET>(args, W, basis, gref, acc, racc, lrows, lbase, nr, g, ok, el, idx);
+        gather_level1<P>(J, min(d.e0 + b + el, vlast), idx, g);
+        gather_level2<P>(J, idx, g);
+        owner_batch<P, T, RES, JAC, DET>(mat, W, basis, gref, acc, racc, lrows, lbase, d.nr, g, ok, el, idx);
       }
       __syncthreads();
     }
   } else {
-    // Software pipeline: gathers for visit batch k+1 (coordinates, states, ranks) and k+2 (node ids)
-    // are in flight while batch k is computed.
-    for (; vcur - el < ne; vcur += STEP) {
-      const bool ok = laneOk && vcur < ne;
+    // Software pipeline: gathers for batch k+1 (coordinates, states, ranks) and k+2 (node ids) are in
+    // flight while batch k is computed. Loads are unconditional, indices clamped into the part.
+    int vcur = wave * O::EPW + el;
+    Gather<P> cur, nxt;
+    gather_level1<P>(J, min(d.e0 + vcur, vlast), idx, cur);
+    gather_level2<P>(J, idx, cur);
+    gather_level1<P>(J, min(d.e0 + vcur + STEP, vlast), idx, nxt);
+    for (; vcur - el < d.ne; vcur += STEP) {
+      const bool ok = laneOk && vcur < d.ne;
       const Gather<P> g = cur;
       cur = nxt;
-      gather_level2<P>(args, idx, cur);
-      gather_level1<P>(args, min(e0 + vcur + 2 * STEP, vlast), idx, nxt);
-      owner_batch<P, T, RES, JAC, DET>(args, W, basis, gref, acc, racc, lrows, lbase, nr, g, ok, el, idx);
+      gather_level2<P>(J, idx, cur);
+      gather_level1<P>(J, min(d.e0 + vcur + 2 * STEP, vlast), idx, nxt);
+      owner_batch<P, T, RES, JAC, DET>(mat, W, basis, gref, acc, racc, lrows, lbase, d.nr, g, ok, el, idx);
     }
     __syncthreads();
   }
-
-  // Write-once output: each BSR run is contiguous in LDS and in the value array; one wave per run.
-  if (JAC) {
-    for (int r = wave; r < nruns; r += WAVES) {
-      const int4 run = lruns[r];
-      const double2 *src = reinterpret_cast<const double2 *>(acc) + 2 * run.x;
-      double2 *dst = reinterpret_cast<double2 *>(args.values) + 2 * int64_t(run.y);
-      for (int i = lane; i < 2 * run.z; i += 64) dst[i] = src[i];
-    }
-  }
-  if (RES) {
-    for (int i = threadIdx.x; i < nr; i += blockDim.x) {
-      reinterpret_cast<double2 *>(args.residual)[lrows[i]] = reinterpret_cast<const double2 *>(racc)[i];
-    }
-  }
+  owner_write<RES, JAC, WAVES>(J, acc, racc, lruns, d.nruns, lrows, d.nr);
 }
 
-// Streaming variant for plans whose parts need one batch per wavefront (the default partition):
-// a persistent workgroup walks a contiguous range of parts and, while part k is computed, loads
-// part k+1's row tables and element gathers and part k+2's descriptor. Only the accumulator set-up
-// and the write-out separate two parts.
+// Streaming kernel for parts that take one batch per wavefront (the default partition), over one or
+// many jobs (a batch of resident meshes of one order). A persistent workgroup walks a contiguous range
+// of global parts; while part k is computed it loads part k+1's row tables and element gathers and
+// part k+2's descriptor. Only the accumulator set-up and the write-out separate two parts.
 struct PartTables {
   int row, lbase;
   int runLds, runGlobal, runLen;
 };
 
-__device__ __forceinline__ void load_part_desc(const OwnerArgs &a, int part, int4 &d0, int4 &d1) {
-  d0 = a.partDesc[2 * part];
-  d1 = a.partDesc[2 * part + 1];
-}
-
-__device__ __forceinline__ void load_part_tables(const OwnerArgs &a, const int4 &d0, const int4 &d1, int tid,
-                                                 PartTables &t) {
-  const int i = min(tid, d0.y - 1);
-  t.row = a.rows[d0.x + i];
-  t.lbase = a.rowLbase[d0.x + i];
-  const int4 r = a.runs[d0.w + min(tid, d1.x - 1)];
+__device__ __forceinline__ void load_part_tables(const MeshJob &J, const PartDesc &d, int tid, PartTables &t) {
+  const int i = min(tid, d.nr - 1);
+  t.row = J.rows[d.r0 + i];
+  t.lbase = J.rowLbase[d.r0 + i];
+  const int4 r = J.runs[d.rb + min(tid, d.nruns - 1)];
   t.runLds = r.x;
   t.runGlobal = r.y;
   t.runLen = r.z;
+}
+
+// Job index of global part gp, searching forward from job j (parts of a job are contiguous).
+__device__ __forceinline__ int advance_job(const OwnerArgs &a, int j, int gp) {
+  while (j + 1 < a.numJobs && a.jobStart[j + 1] <= gp) j++;
+  return j;
 }
 
 template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC>
@@ -569,10 +581,10 @@ __global__ __launch_bounds__(WAVES * 64) void owner_stream_kernel(const OwnerArg
   __shared__ int lbase[MAXR];
   __shared__ int4 lruns[JAC ? MAXR : 1];
 
-  // Contiguous part range per workgroup; workgroup w runs on XCD w % 8, and each XCD's workgroups
-  // share one contiguous (spatially adjacent, Morton-ordered) stretch of parts.
+  // Contiguous range of global parts per workgroup; workgroup w runs on XCD w % 8, and each XCD's
+  // workgroups share one contiguous (spatially adjacent, Morton-ordered) stretch of parts.
   const int xcd = int(blockIdx.x % 8), slot = int(blockIdx.x / 8), perX = int(gridDim.x / 8);
-  const int64_t np = args.numParts;
+  const int64_t np = args.totalParts;
   const int xb = int(np * xcd / 8), xe = int(np * (xcd + 1) / 8);
   const int pb = xb + int(int64_t(xe - xb) * slot / perX);
   const int pe = xb + int(int64_t(xe - xb) * (slot + 1) / perX);
@@ -584,60 +596,64 @@ __global__ __launch_bounds__(WAVES * 64) void owner_stream_kernel(const OwnerArg
   const int myv = wave * O::EPW + el;  // this lane's element visit within a part
   WaveLDS<P, T> &W = wl[wave];
 
-  int4 c0, c1, n0, n1, nn0, nn1;
+  // first job: binary search of the job start table
+  int jc = 0;
+  {
+    int lo = 0, hi = args.numJobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (args.jobStart[mid] <= pb) lo = mid;
+      else hi = mid - 1;
+    }
+    jc = lo;
+  }
+  PartDesc dc = part_desc(args.jobs[jc], pb - args.jobStart[jc]);
   PartTables tc, tn;
   Gather<P> gc, gn;
-  load_part_desc(args, pb, c0, c1);
-  load_part_tables(args, c0, c1, tid, tc);
-  gather_level1<P>(args, c1.y + min(myv, c1.z - 1), idx, gc);
-  gather_level2<P>(args, idx, gc);
-  load_part_desc(args, min(pb + 1, pe - 1), n0, n1);
+  load_part_tables(args.jobs[jc], dc, tid, tc);
+  gather_level1<P>(args.jobs[jc], dc.e0 + min(myv, dc.ne - 1), idx, gc);
+  gather_level2<P>(args.jobs[jc], idx, gc);
+  int jn = advance_job(args, jc, min(pb + 1, pe - 1));
+  PartDesc dn = part_desc(args.jobs[jn], min(pb + 1, pe - 1) - args.jobStart[jn]);
   copy_basis(basis, args.basis);
   fill_gref<P>(gref, args.basis);
 
-  for (int part = pb; part < pe; part++) {
+  for (int gp = pb; gp < pe; gp++) {
     __syncthreads();  // the previous part's write-out has read the accumulators
-    if (tid < c0.y) {
+    if (tid < dc.nr) {
       lrows[tid] = tc.row;
       lbase[tid] = tc.lbase;
     }
     if (JAC) {
-      if (tid < c1.x) lruns[tid] = make_int4(tc.runLds, tc.runGlobal, tc.runLen, 0);
-      for (int i = tid; i < 2 * c0.z; i += blockDim.x) reinterpret_cast<double2 *>(acc)[i] = make_double2(0, 0);
+      if (tid < dc.nruns) lruns[tid] = make_int4(tc.runLds, tc.runGlobal, tc.runLen, 0);
+      for (int i = tid; i < 2 * dc.nb; i += blockDim.x) reinterpret_cast<double2 *>(acc)[i] = make_double2(0, 0);
     }
     if (RES) {
-      for (int i = tid; i < c0.y; i += blockDim.x) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
+      for (int i = tid; i < dc.nr; i += blockDim.x) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
     }
     __syncthreads();
 
-    // Prefetch: part+1 tables and node/element ids, part+2 descriptor (indices clamped to the range).
-    load_part_tables(args, n0, n1, tid, tn);
-    gather_level1<P>(args, n1.y + min(myv, n1.z - 1), idx, gn);
-    load_part_desc(args, min(part + 2, pe - 1), nn0, nn1);
+    // Prefetch: part gp+1 tables and node/element ids, part gp+2 descriptor (clamped to the range).
+    const MeshJob &Jn = args.jobs[jn];
+    load_part_tables(Jn, dn, tid, tn);
+    gather_level1<P>(Jn, dn.e0 + min(myv, dn.ne - 1), idx, gn);
+    const int gnn = min(gp + 2, pe - 1);
+    const int jnn = advance_job(args, jn, gnn);
+    const PartDesc dnn = part_desc(args.jobs[jnn], gnn - args.jobStart[jnn]);
 
-    const bool ok = laneOk && myv < c1.z;
-    owner_batch<P, T, RES, JAC, false>(args, W, basis, gref, acc, racc, lrows, lbase, c0.y, gc, ok, el, idx);
+    const MeshJob &Jc = args.jobs[jc];
+    const bool ok = laneOk && myv < dc.ne;
+    owner_batch<P, T, RES, JAC, false>(job_material(Jc), W, basis, gref, acc, racc, lrows, lbase, dc.nr, gc, ok, el,
+                                       idx);
 
-    gather_level2<P>(args, idx, gn);  // part+1 coordinates, states, ranks
+    gather_level2<P>(Jn, idx, gn);  // part gp+1 coordinates, states, ranks
     __syncthreads();
+    owner_write<RES, JAC, WAVES>(Jc, acc, racc, lruns, dc.nruns, lrows, dc.nr);
 
-    if (JAC) {
-      for (int r = wave; r < c1.x; r += WAVES) {
-        const int4 run = lruns[r];
-        const double2 *src = reinterpret_cast<const double2 *>(acc) + 2 * run.x;
-        double2 *dst = reinterpret_cast<double2 *>(args.values) + 2 * int64_t(run.y);
-        for (int i = lane; i < 2 * run.z; i += 64) dst[i] = src[i];
-      }
-    }
-    if (RES) {
-      for (int i = tid; i < c0.y; i += blockDim.x) {
-        reinterpret_cast<double2 *>(args.residual)[lrows[i]] = reinterpret_cast<const double2 *>(racc)[i];
-      }
-    }
-    c0 = n0;
-    c1 = n1;
-    n0 = nn0;
-    n1 = nn1;
+    jc = jn;
+    dc = dn;
+    jn = jnn;
+    dn = dnn;
     tc = tn;
     gc = gn;
   }
@@ -753,48 +769,108 @@ void launch_element_order(int order, const ElemArgs &a, int precision, hipStream
   }
 }
 
-// Streaming (persistent) launches: workgroups resident at once on the device.
-int g_streamGrid = 0;
+// One launch group: jobs of one order with the same outputs.
+struct OwnerLaunch {
+  OwnerArgs args;
+  int precision;
+  bool det;         // deterministic (colour-phased) sums: per-part kernel
+  bool res, jac;    // outputs
+  bool rOnlyParts;  // parts sized for residual-only assembly
+  bool stream;      // every part is one batch per wavefront: streaming kernel
+  int streamGrid;   // persistent workgroups (2 per CU, multiple of 8)
+};
 
 template <int P, typename T, int MAXR, bool RES, bool JAC, bool DET>
-void launch_owner(const OwnerArgs &a, hipStream_t stream) {
+void launch_owner(const OwnerLaunch &L, hipStream_t stream) {
   constexpr OwnerConfig cfg = owner_config(P);
   if constexpr (!DET && MAXR <= cfg.waves * 64) {
-    if (g_streamGrid > 0) {
-      const int grid = std::min(g_streamGrid, ((a.numParts + 7) / 8) * 8);
-      owner_stream_kernel<P, T, cfg.waves, cfg.maxBlocks, MAXR, RES, JAC><<<grid, cfg.waves * 64, 0, stream>>>(a);
+    if (L.stream) {
+      const int grid = std::min(L.streamGrid, ((L.args.totalParts + 7) / 8) * 8);
+      owner_stream_kernel<P, T, cfg.waves, cfg.maxBlocks, MAXR, RES, JAC><<<grid, cfg.waves * 64, 0, stream>>>(
+          L.args);
       return;
     }
   }
-  const int grid = ((a.numParts + 7) / 8) * 8;
-  owner_kernel<P, T, cfg.waves, cfg.maxBlocks, MAXR, RES, JAC, DET><<<grid, cfg.waves * 64, 0, stream>>>(a);
+  const int grid = ((L.args.totalParts + 7) / 8) * 8;
+  owner_kernel<P, T, cfg.waves, cfg.maxBlocks, MAXR, RES, JAC, DET><<<grid, cfg.waves * 64, 0, stream>>>(L.args);
 }
 
 template <int P, typename T, bool DET>
-void launch_owner_mode(const OwnerArgs &a, bool res, bool jac, bool residualOnlyParts, hipStream_t stream) {
+void launch_owner_mode(const OwnerLaunch &L, hipStream_t stream) {
   constexpr OwnerConfig cfg = owner_config(P);
-  if (jac && res) {
-    launch_owner<P, T, cfg.maxRows, true, true, DET>(a, stream);
-  } else if (jac) {
-    launch_owner<P, T, cfg.maxRows, false, true, DET>(a, stream);
-  } else if (residualOnlyParts) {
-    launch_owner<P, T, kResidualOnlyMaxRows, true, false, DET>(a, stream);
+  if (L.jac && L.res) {
+    launch_owner<P, T, cfg.maxRows, true, true, DET>(L, stream);
+  } else if (L.jac) {
+    launch_owner<P, T, cfg.maxRows, false, true, DET>(L, stream);
+  } else if (L.rOnlyParts) {
+    launch_owner<P, T, kResidualOnlyMaxRows, true, false, DET>(L, stream);
   } else {
     // Residual from a plan whose parts were sized for the Jacobian: rows fit the smaller bound.
-    launch_owner<P, T, cfg.maxRows, true, false, DET>(a, stream);
+    launch_owner<P, T, cfg.maxRows, true, false, DET>(L, stream);
   }
 }
 
 template <int P>
-void launch_owner_order(const OwnerArgs &a, int precision, bool det, bool res, bool jac, bool rOnly,
-                        hipStream_t stream) {
-  if (precision == FEM_PREC_MIXED) {
-    if (det) launch_owner_mode<P, float, true>(a, res, jac, rOnly, stream);
-    else launch_owner_mode<P, float, false>(a, res, jac, rOnly, stream);
+void launch_owner_order(const OwnerLaunch &L, hipStream_t stream) {
+  if (L.precision == FEM_PREC_MIXED) {
+    if (L.det) launch_owner_mode<P, float, true>(L, stream);
+    else launch_owner_mode<P, float, false>(L, stream);
   } else {
-    if (det) launch_owner_mode<P, double, true>(a, res, jac, rOnly, stream);
-    else launch_owner_mode<P, double, false>(a, res, jac, rOnly, stream);
+    if (L.det) launch_owner_mode<P, double, true>(L, stream);
+    else launch_owner_mode<P, double, false>(L, stream);
   }
+}
+
+void launch_owner_any(int order, const OwnerLaunch &L, hipStream_t stream) {
+  switch (order) {
+    case 1: launch_owner_order<1>(L, stream); break;
+    case 2: launch_owner_order<2>(L, stream); break;
+    case 3: launch_owner_order<3>(L, stream); break;
+    default: launch_owner_order<4>(L, stream); break;
+  }
+}
+
+int owner_waves(int order) {
+  switch (order) {
+    case 1: return owner_config(1).waves;
+    case 2: return owner_config(2).waves;
+    case 3: return owner_config(3).waves;
+    default: return owner_config(4).waves;
+  }
+}
+
+// Whether a plan's parts all fit one batch per wavefront of the owner workgroup.
+bool plan_streams(const fem_plan *p) {
+  const bool rOnly = (p->flags & FEM_PLAN_RESIDUAL_ONLY) != 0;
+  const bool det = (p->flags & FEM_PLAN_DETERMINISTIC) != 0;
+  const int epw = 64 / nodes_per_element(p->order);
+  return !rOnly && !det && p->numCUs > 0 && p->maxPartVisits <= owner_waves(p->order) * epw;
+}
+
+int stream_grid(int numCUs) { return ((2 * numCUs + 7) / 8) * 8; }
+
+MeshJob make_job(const fem_plan *p, const double *states, const double *coords, double *residual, double *values,
+                 const Material &m) {
+  MeshJob j{};
+  j.visitConn = p->dVisitConn;
+  j.visitEid = p->dVisitEid;
+  j.nodeLrow = p->dNodeLrow;
+  j.rows = p->dRows;
+  j.rowLbase = p->dRowLbase;
+  j.runs = reinterpret_cast<const int4 *>(p->dRuns);
+  j.partDesc = reinterpret_cast<const int4 *>(p->dPartDesc);
+  j.rank = reinterpret_cast<const uint32_t *>(p->dRank);
+  j.colourOff = p->dColourOff;
+  j.states = states;
+  j.coords = coords;
+  j.residual = residual;
+  j.values = values;
+  j.mu = m.mu;
+  j.lambda = m.lambda;
+  j.t = m.t;
+  j.numParts = p->numParts;
+  j.numColours = p->numColours;
+  return j;
 }
 
 int check_common(int order, const int *connPtr, const int *conn, int numElements, const double *states,
@@ -820,9 +896,8 @@ int upload(const std::vector<TPtr> &h, TPtr **d, int64_t &bytes) {
 }
 
 void free_plan_arrays(fem_plan *p) {
-  void *ptrs[] = {p->dVisitConn,  p->dVisitEid, p->dNodeLrow,    p->dPartRowPtr, p->dRows,
-                  p->dRowLbase,   p->dPartBlocks, p->dPartRunPtr, p->dRuns,       p->dPartElemPtr,
-                  p->dColourOff,  p->dPartDesc,   p->dRank};
+  void *ptrs[] = {p->dVisitConn, p->dVisitEid, p->dNodeLrow, p->dRows,     p->dRowLbase, p->dRuns,
+                  p->dColourOff, p->dPartDesc, p->dRank,     p->dJob,      p->dJobStart};
   for (void *q : ptrs) {
     if (q) (void)hipFree(q);
   }
@@ -945,18 +1020,24 @@ int fem_plan_create(int order, int numNodes, int numElements, const int *connPtr
   p->maxPartVisits = h.maxPartVisits;
   (void)hipGetDevice(&p->device);
   if (hipDeviceGetAttribute(&p->numCUs, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess) p->numCUs = 0;
+  const std::vector<int> jobStart = {0, h.numParts};
   int64_t bytes = 0;
   if ((rc = upload(h.visitConn, &p->dVisitConn, bytes)) || (rc = upload(h.visitEid, &p->dVisitEid, bytes)) ||
-      (rc = upload(h.nodeLrow, &p->dNodeLrow, bytes)) || (rc = upload(h.partRowPtr, &p->dPartRowPtr, bytes)) ||
-      (rc = upload(h.rows, &p->dRows, bytes)) || (rc = upload(h.rowLbase, &p->dRowLbase, bytes)) ||
-      (rc = upload(h.partBlocks, &p->dPartBlocks, bytes)) || (rc = upload(h.partRunPtr, &p->dPartRunPtr, bytes)) ||
-      (rc = upload(h.runs, &p->dRuns, bytes)) || (rc = upload(h.partElemPtr, &p->dPartElemPtr, bytes)) ||
+      (rc = upload(h.nodeLrow, &p->dNodeLrow, bytes)) || (rc = upload(h.rows, &p->dRows, bytes)) ||
+      (rc = upload(h.rowLbase, &p->dRowLbase, bytes)) || (rc = upload(h.runs, &p->dRuns, bytes)) ||
       (rc = upload(h.colourOff, &p->dColourOff, bytes)) || (rc = upload(h.partDesc, &p->dPartDesc, bytes)) ||
-      (rc = upload(h.rank, &p->dRank, bytes))) {
+      (rc = upload(h.rank, &p->dRank, bytes)) || (rc = upload(jobStart, &p->dJobStart, bytes))) {
     free_plan_arrays(p);
     delete p;
     return rc;
   }
+  if (hipMalloc(reinterpret_cast<void **>(&p->dJob), sizeof(MeshJob)) != hipSuccess) {
+    free_plan_arrays(p);
+    delete p;
+    return set_error(FEM_ERR_OUT_OF_MEMORY, "hipMalloc of the plan's job record failed");
+  }
+  bytes += int64_t(sizeof(MeshJob));
+  static_assert(sizeof(MeshJob) <= sizeof(p->hostJob), "job record size");
   p->deviceBytes = bytes;
   *plan = p;
   return FEM_OK;
@@ -984,47 +1065,32 @@ int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCo
   if (rOnly && matEntries) {
     return set_error(FEM_ERR_INVALID_ARGUMENT, "plan was built residual-only; it cannot assemble the Jacobian");
   }
-  OwnerArgs a{};
-  a.visitConn = plan->dVisitConn;
-  a.visitEid = plan->dVisitEid;
-  a.nodeLrow = plan->dNodeLrow;
-  a.partRowPtr = plan->dPartRowPtr;
-  a.rows = plan->dRows;
-  a.rowLbase = plan->dRowLbase;
-  a.partBlocks = plan->dPartBlocks;
-  a.partRunPtr = plan->dPartRunPtr;
-  a.runs = reinterpret_cast<const int4 *>(plan->dRuns);
-  a.partElemPtr = plan->dPartElemPtr;
-  a.colourOff = plan->dColourOff;
-  a.rank = reinterpret_cast<const uint32_t *>(plan->dRank);
-  a.partDesc = reinterpret_cast<const int4 *>(plan->dPartDesc);
-  a.numParts = plan->numParts;
-  a.numColours = plan->numColours;
-  a.states = states;
-  a.coords = nodeCoords;
-  a.residual = residual;
-  a.values = matEntries;
-  int rc;
-  if ((rc = make_basis(plan->order, a.basis)) || (rc = make_material(E, nu, t, a.mat))) return rc;
-  const bool det = (plan->flags & FEM_PLAN_DETERMINISTIC) != 0;
+  Material m;
+  int rc = make_material(E, nu, t, m);
+  if (rc) return rc;
+  OwnerLaunch L{};
+  if ((rc = make_basis(plan->order, L.args.basis))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  // Persistent streaming kernel when every part is one batch per wavefront (two workgroups per CU).
-  int waves = 4, epw = 64 / nodes_per_element(plan->order);
-  switch (plan->order) {
-    case 1: waves = owner_config(1).waves; break;
-    case 2: waves = owner_config(2).waves; break;
-    case 3: waves = owner_config(3).waves; break;
-    default: waves = owner_config(4).waves; break;
+  // The device job record is rewritten only when the request's pointers or material change.
+  const MeshJob job = make_job(plan, states, nodeCoords, residual, matEntries, m);
+  if (!plan->jobUploaded || std::memcmp(&job, plan->hostJob, sizeof(MeshJob)) != 0) {
+    FEM_HIP_CHECK(hipMemcpyAsync(plan->dJob, &job, sizeof(MeshJob), hipMemcpyHostToDevice, s));
+    std::memcpy(plan->hostJob, &job, sizeof(MeshJob));
+    plan->jobUploaded = true;
   }
-  const bool stream1 = !rOnly && plan->numCUs > 0 && plan->maxPartVisits <= waves * epw;
-  g_streamGrid = stream1 ? ((2 * plan->numCUs + 7) / 8) * 8 : 0;
+  L.args.jobs = reinterpret_cast<const MeshJob *>(plan->dJob);
+  L.args.jobStart = plan->dJobStart;
+  L.args.numJobs = 1;
+  L.args.totalParts = plan->numParts;
+  L.precision = precision;
+  L.det = (plan->flags & FEM_PLAN_DETERMINISTIC) != 0;
+  L.res = residual != nullptr;
+  L.jac = matEntries != nullptr;
+  L.rOnlyParts = rOnly;
+  L.stream = plan_streams(plan);
+  L.streamGrid = stream_grid(plan->numCUs);
   return timed(s, elapsedSeconds, [&]() {
-    switch (plan->order) {
-      case 1: launch_owner_order<1>(a, precision, det, residual != nullptr, matEntries != nullptr, rOnly, s); break;
-      case 2: launch_owner_order<2>(a, precision, det, residual != nullptr, matEntries != nullptr, rOnly, s); break;
-      case 3: launch_owner_order<3>(a, precision, det, residual != nullptr, matEntries != nullptr, rOnly, s); break;
-      default: launch_owner_order<4>(a, precision, det, residual != nullptr, matEntries != nullptr, rOnly, s); break;
-    }
+    launch_owner_any(plan->order, L, s);
     return FEM_OK;
   });
 }
@@ -1033,6 +1099,148 @@ int fem_plan_destroy(fem_plan *plan) {
   if (!plan) return FEM_OK;
   free_plan_arrays(plan);
   delete plan;
+  return FEM_OK;
+}
+
+// ---- batches of resident plans ------------------------------------------------------------------
+}  // extern "C"
+
+struct fem_batch {
+  struct Group {
+    std::vector<MeshJob> jobs;
+    std::vector<int> jobStart{0};
+    MeshJob *dJobs = nullptr;
+    int *dJobStart = nullptr;
+    size_t capJobs = 0;
+    bool stream = true;
+  };
+  // group index = (order-1)*3 + outputs (0: residual, 1: values, 2: both)
+  Group groups[12];
+  int numCUs = 0;
+  int precision = FEM_PREC_FP64;
+  int numJobs = 0;
+};
+
+namespace {
+void free_batch_group(fem_batch::Group &g) {
+  if (g.dJobs) (void)hipFree(g.dJobs);
+  if (g.dJobStart) (void)hipFree(g.dJobStart);
+  g.dJobs = nullptr;
+  g.dJobStart = nullptr;
+  g.capJobs = 0;
+}
+}  // namespace
+
+extern "C" {
+
+int fem_batch_create(fem_batch **batch) {
+  if (!batch) return set_error(FEM_ERR_NULL_POINTER, "batch is NULL");
+  fem_batch *b = new fem_batch();
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&b->numCUs, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) b->numCUs = 0;
+  *batch = b;
+  return FEM_OK;
+}
+
+int fem_batch_clear(fem_batch *batch) {
+  if (!batch) return set_error(FEM_ERR_NULL_POINTER, "batch is NULL");
+  for (auto &g : batch->groups) {
+    g.jobs.clear();
+    g.jobStart.assign(1, 0);
+    g.stream = true;
+  }
+  batch->numJobs = 0;
+  return FEM_OK;
+}
+
+int fem_batch_add(fem_batch *batch, fem_plan *plan, const double *states, const double *nodeCoords, double E,
+                  double nu, double t, double *residual, double *matEntries) {
+  if (!batch || !plan) return set_error(FEM_ERR_NULL_POINTER, "batch/plan is NULL");
+  if (!states || !nodeCoords) return set_error(FEM_ERR_NULL_POINTER, "states/nodeCoords is NULL");
+  if (!residual && !matEntries) return set_error(FEM_ERR_NULL_POINTER, "residual and matEntries are both NULL");
+  if ((plan->flags & FEM_PLAN_RESIDUAL_ONLY) && matEntries) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "plan was built residual-only; it cannot assemble the Jacobian");
+  }
+  if ((plan->flags & FEM_PLAN_DETERMINISTIC) || (plan->flags & FEM_PLAN_RESIDUAL_ONLY)) {
+    return set_error(FEM_ERR_UNSUPPORTED, "batches take plans built with default flags");
+  }
+  Material m;
+  int rc = make_material(E, nu, t, m);
+  if (rc) return rc;
+  const int outputs = (residual && matEntries) ? 2 : (matEntries ? 1 : 0);
+  fem_batch::Group &g = batch->groups[(plan->order - 1) * 3 + outputs];
+  g.jobs.push_back(make_job(plan, states, nodeCoords, residual, matEntries, m));
+  g.jobStart.push_back(g.jobStart.back() + plan->numParts);
+  g.stream = g.stream && plan_streams(plan);
+  batch->numJobs++;
+  return FEM_OK;
+}
+
+int fem_batch_assemble(fem_batch *batch, int precision, void *stream, double *elapsedSeconds) {
+  if (!batch) return set_error(FEM_ERR_NULL_POINTER, "batch is NULL");
+  if (precision != FEM_PREC_FP64 && precision != FEM_PREC_MIXED) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "unknown precision mode %d", precision);
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // Upload every non-empty group's job table before the timed launches.
+  for (auto &g : batch->groups) {
+    if (g.jobs.empty()) continue;
+    if (g.capJobs < g.jobs.size()) {
+      free_batch_group(g);
+      const size_t cap = std::max<size_t>(g.jobs.size(), 16);
+      if (hipMalloc(reinterpret_cast<void **>(&g.dJobs), cap * sizeof(MeshJob)) != hipSuccess ||
+          hipMalloc(reinterpret_cast<void **>(&g.dJobStart), (cap + 1) * sizeof(int)) != hipSuccess) {
+        free_batch_group(g);
+        return set_error(FEM_ERR_OUT_OF_MEMORY, "hipMalloc of batch job tables failed");
+      }
+      g.capJobs = cap;
+    }
+    FEM_HIP_CHECK(hipMemcpyAsync(g.dJobs, g.jobs.data(), g.jobs.size() * sizeof(MeshJob), hipMemcpyHostToDevice, s));
+    FEM_HIP_CHECK(hipMemcpyAsync(g.dJobStart, g.jobStart.data(), g.jobStart.size() * sizeof(int),
+                                 hipMemcpyHostToDevice, s));
+  }
+  return timed(s, elapsedSeconds, [&]() -> int {
+    for (int gi = 0; gi < 12; gi++) {
+      fem_batch::Group &g = batch->groups[gi];
+      if (g.jobs.empty()) continue;
+      const int order = gi / 3 + 1, outputs = gi % 3;
+      OwnerLaunch L{};
+      int rc = make_basis(order, L.args.basis);
+      if (rc) return rc;
+      L.args.jobs = g.dJobs;
+      L.args.jobStart = g.dJobStart;
+      L.args.numJobs = int(g.jobs.size());
+      L.args.totalParts = g.jobStart.back();
+      L.precision = precision;
+      L.det = false;
+      L.res = outputs != 1;
+      L.jac = outputs != 0;
+      L.rOnlyParts = false;
+      L.stream = g.stream && batch->numCUs > 0;
+      L.streamGrid = stream_grid(batch->numCUs);
+      if (L.stream) {
+        launch_owner_any(order, L, s);
+      } else {
+        // Parts too large for streaming: one per-part launch per job.
+        for (size_t j = 0; j < g.jobs.size(); j++) {
+          OwnerLaunch Lj = L;
+          Lj.args.jobs = g.dJobs + j;
+          Lj.args.jobStart = g.dJobStart;  // only [0] is read for a single job: first part 0 of the job
+          Lj.args.numJobs = 1;
+          Lj.args.totalParts = g.jobs[j].numParts;
+          launch_owner_any(order, Lj, s);
+        }
+      }
+    }
+    return FEM_OK;
+  });
+}
+
+int fem_batch_destroy(fem_batch *batch) {
+  if (!batch) return FEM_OK;
+  for (auto &g : batch->groups) free_batch_group(g);
+  delete batch;
   return FEM_OK;
 }
 

@@ -84,16 +84,16 @@ struct fem_plan {
   int *dVisitConn = nullptr;
   int *dVisitEid = nullptr;
   int *dNodeLrow = nullptr;
-  int *dPartRowPtr = nullptr;
   int *dRows = nullptr;
   int *dRowLbase = nullptr;
-  int *dPartBlocks = nullptr;
-  int *dPartRunPtr = nullptr;
   int *dRuns = nullptr;
-  int *dPartElemPtr = nullptr;
   int *dColourOff = nullptr;
   int *dPartDesc = nullptr;
   uint8_t *dRank = nullptr;
+  int *dJobStart = nullptr;  // {0, numParts}
+  void *dJob = nullptr;      // device job record of the last assembly
+  alignas(8) unsigned char hostJob[256] = {};  // host copy of that record
+  bool jobUploaded = false;
   int maxPartVisits = 0;
   int numCUs = 0;
 };

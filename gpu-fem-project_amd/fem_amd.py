@@ -71,6 +71,11 @@ SIGNATURES = {
     "fem_plan_assemble": ([_P, _P, _P, _D, _D, _D, _P, _P, _I, _P, ctypes.POINTER(_D)], _I),
     "fem_plan_destroy": ([_P], _I),
     "fem_bsr_spmv": ([_I, _P, _P, _P, _P, _P, _P], _I),
+    "fem_batch_create": ([ctypes.POINTER(_P)], _I),
+    "fem_batch_clear": ([_P], _I),
+    "fem_batch_add": ([_P, _P, _P, _P, _D, _D, _D, _P, _P], _I),
+    "fem_batch_assemble": ([_P, _I, _P, ctypes.POINTER(_D)], _I),
+    "fem_batch_destroy": ([_P], _I),
 }
 
 _lib = None
@@ -338,6 +343,44 @@ class Plan:
     def close(self):
         if self.handle:
             self._lib.fem_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many assemblies on resident plans in one call (fem_batch_*): one persistent launch per element
+    order and output set."""
+
+    def __init__(self):
+        self._lib = load()
+        handle = ctypes.c_void_p()
+        _check(self._lib.fem_batch_create(ctypes.byref(handle)))
+        self.handle = handle
+        self._keep = []
+
+    def clear(self):
+        _check(self._lib.fem_batch_clear(self.handle))
+        self._keep = []
+
+    def add(self, plan: Plan, states, coords, E, nu, t, res=None, values=None):
+        _check(self._lib.fem_batch_add(self.handle, plan.handle, _dptr(states), _dptr(coords), E, nu, t, _dptr(res),
+                                       _dptr(values)))
+        self._keep.append((plan, states, coords, res, values))
+
+    def assemble(self, precision=FEM_PREC_FP64, stream=None, timed=False):
+        sec = ctypes.c_double()
+        _check(self._lib.fem_batch_assemble(self.handle, precision, _stream(stream),
+                                            ctypes.byref(sec) if timed else None))
+        return sec.value if timed else None
+
+    def close(self):
+        if self.handle:
+            self._lib.fem_batch_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

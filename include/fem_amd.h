@@ -151,6 +151,20 @@ int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCo
                       double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds);
 int fem_plan_destroy(fem_plan *plan);
 
+/* ---- batches of resident plans ----------------------------------------------------------------
+ * Many assemblies on many resident plans in one call: jobs are grouped by element order and outputs
+ * and each group runs as ONE persistent launch over all of its meshes' parts (no per-mesh launches).
+ * Pointers are the same as fem_plan_assemble's; outputs are overwritten. Plans must be built with
+ * flags 0. fem_batch_assemble uploads the job tables (asynchronously on `stream`) and launches; the
+ * batch may be cleared and refilled once the call returns. */
+typedef struct fem_batch fem_batch;
+int fem_batch_create(fem_batch **batch);
+int fem_batch_clear(fem_batch *batch);
+int fem_batch_add(fem_batch *batch, fem_plan *plan, const double *states, const double *nodeCoords, double E,
+                  double nu, double t, double *residual, double *matEntries);
+int fem_batch_assemble(fem_batch *batch, int precision, void *stream, double *elapsedSeconds);
+int fem_batch_destroy(fem_batch *batch);
+
 /* ---- BSR utilities (device) -------------------------------------------------------------------
  * y = A x for the 2x2-block BSR matrix (used to check R = K u). */
 int fem_bsr_spmv(int numRows, const int *rowp, const int *cols, const double *values, const double *x, double *y,
