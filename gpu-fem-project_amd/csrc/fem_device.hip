@@ -358,6 +358,7 @@ struct MeshJob {
 struct OwnerArgs {
   const MeshJob *jobs;    // device job table
   const int *jobStart;    // [numJobs+1] first global part of each job
+  const int *counts;      // if non-NULL: device (numJobs, totalParts), read instead of the two fields below
   int numJobs;
   int totalParts;
   Basis1D basis;
@@ -563,8 +564,8 @@ __device__ __forceinline__ void load_part_tables(const MeshJob &J, const PartDes
 }
 
 // Job index of global part gp, searching forward from job j (parts of a job are contiguous).
-__device__ __forceinline__ int advance_job(const OwnerArgs &a, int j, int gp) {
-  while (j + 1 < a.numJobs && a.jobStart[j + 1] <= gp) j++;
+__device__ __forceinline__ int advance_job(const OwnerArgs &a, int numJobs, int j, int gp) {
+  while (j + 1 < numJobs && a.jobStart[j + 1] <= gp) j++;
   return j;
 }
 
@@ -584,7 +585,8 @@ __global__ __launch_bounds__(WAVES * 64) void owner_stream_kernel(const OwnerArg
   // Contiguous range of global parts per workgroup; workgroup w runs on XCD w % 8, and each XCD's
   // workgroups share one contiguous (spatially adjacent, Morton-ordered) stretch of parts.
   const int xcd = int(blockIdx.x % 8), slot = int(blockIdx.x / 8), perX = int(gridDim.x / 8);
-  const int64_t np = args.totalParts;
+  const int numJobs = args.counts ? args.counts[0] : args.numJobs;
+  const int64_t np = args.counts ? args.counts[1] : args.totalParts;
   const int xb = int(np * xcd / 8), xe = int(np * (xcd + 1) / 8);
   const int pb = xb + int(int64_t(xe - xb) * slot / perX);
   const int pe = xb + int(int64_t(xe - xb) * (slot + 1) / perX);
@@ -599,7 +601,7 @@ __global__ __launch_bounds__(WAVES * 64) void owner_stream_kernel(const OwnerArg
   // first job: binary search of the job start table
   int jc = 0;
   {
-    int lo = 0, hi = args.numJobs - 1;
+    int lo = 0, hi = numJobs - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) / 2;
       if (args.jobStart[mid] <= pb) lo = mid;
@@ -613,7 +615,7 @@ __global__ __launch_bounds__(WAVES * 64) void owner_stream_kernel(const OwnerArg
   load_part_tables(args.jobs[jc], dc, tid, tc);
   gather_level1<P>(args.jobs[jc], dc.e0 + min(myv, dc.ne - 1), idx, gc);
   gather_level2<P>(args.jobs[jc], idx, gc);
-  int jn = advance_job(args, jc, min(pb + 1, pe - 1));
+  int jn = advance_job(args, numJobs, jc, min(pb + 1, pe - 1));
   PartDesc dn = part_desc(args.jobs[jn], min(pb + 1, pe - 1) - args.jobStart[jn]);
   copy_basis(basis, args.basis);
   fill_gref<P>(gref, args.basis);
@@ -638,7 +640,7 @@ __global__ __launch_bounds__(WAVES * 64) void owner_stream_kernel(const OwnerArg
     load_part_tables(Jn, dn, tid, tn);
     gather_level1<P>(Jn, dn.e0 + min(myv, dn.ne - 1), idx, gn);
     const int gnn = min(gp + 2, pe - 1);
-    const int jnn = advance_job(args, jn, gnn);
+    const int jnn = advance_job(args, numJobs, jn, gnn);
     const PartDesc dnn = part_desc(args.jobs[jnn], gnn - args.jobStart[jnn]);
 
     const MeshJob &Jc = args.jobs[jc];
@@ -1105,40 +1107,96 @@ int fem_plan_destroy(fem_plan *plan) {
 // ---- batches of resident plans ------------------------------------------------------------------
 }  // extern "C"
 
+// 12 launch groups: (order - 1) * 3 + outputs, outputs 0: residual, 1: values, 2: both.
 struct fem_batch {
   struct Group {
     std::vector<MeshJob> jobs;
     std::vector<int> jobStart{0};
-    MeshJob *dJobs = nullptr;
-    int *dJobStart = nullptr;
-    size_t capJobs = 0;
     bool stream = true;
   };
-  // group index = (order-1)*3 + outputs (0: residual, 1: values, 2: both)
-  Group groups[12];
+  static constexpr int kGroups = 12;
+  Group groups[kGroups];
+  int maxJobs = 0;
+  int flags = 0;
   int numCUs = 0;
-  int precision = FEM_PREC_FP64;
   int numJobs = 0;
+  // Device tables at fixed addresses (a captured graph refers to them).
+  MeshJob *dJobs = nullptr;    // [kGroups][maxJobs]
+  int *dJobStart = nullptr;    // [kGroups][maxJobs + 1]
+  int *dCounts = nullptr;      // [kGroups][4]: numJobs, totalParts
+  std::vector<int> hCounts;    // host image of dCounts
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // per precision
 };
 
 namespace {
-void free_batch_group(fem_batch::Group &g) {
-  if (g.dJobs) (void)hipFree(g.dJobs);
-  if (g.dJobStart) (void)hipFree(g.dJobStart);
-  g.dJobs = nullptr;
-  g.dJobStart = nullptr;
-  g.capJobs = 0;
+
+void free_batch(fem_batch *b) {
+  for (auto &g : b->graph) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+  if (b->dJobs) (void)hipFree(b->dJobs);
+  if (b->dJobStart) (void)hipFree(b->dJobStart);
+  if (b->dCounts) (void)hipFree(b->dCounts);
 }
+
+OwnerLaunch batch_group_launch(const fem_batch *b, int gi, int precision, bool fromDevice) {
+  const fem_batch::Group &g = b->groups[gi];
+  OwnerLaunch L{};
+  make_basis(gi / 3 + 1, L.args.basis);
+  L.args.jobs = b->dJobs + size_t(gi) * b->maxJobs;
+  L.args.jobStart = b->dJobStart + size_t(gi) * (b->maxJobs + 1);
+  L.args.counts = fromDevice ? b->dCounts + 4 * gi : nullptr;
+  L.args.numJobs = int(g.jobs.size());
+  L.args.totalParts = g.jobStart.back();
+  L.precision = precision;
+  L.det = false;
+  L.res = gi % 3 != 1;
+  L.jac = gi % 3 != 0;
+  L.rOnlyParts = false;
+  L.stream = fromDevice || (g.stream && b->numCUs > 0);
+  L.streamGrid = stream_grid(b->numCUs);
+  return L;
+}
+
+// Graph of the 12 group launches, each reading its job count from device memory (empty groups exit).
+int capture_batch_graph(fem_batch *b, int precision, hipStream_t s) {
+  hipGraph_t graph = nullptr;
+  FEM_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  for (int gi = 0; gi < fem_batch::kGroups; gi++) {
+    OwnerLaunch L = batch_group_launch(b, gi, precision, true);
+    L.args.totalParts = L.streamGrid;  // host-side grid bound only; the kernel reads the device count
+    launch_owner_any(gi / 3 + 1, L, s);
+  }
+  FEM_HIP_CHECK(hipStreamEndCapture(s, &graph));
+  hipError_t e = hipGraphInstantiate(&b->graph[precision], graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return set_error(FEM_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+  return FEM_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
-int fem_batch_create(fem_batch **batch) {
+int fem_batch_create(int maxJobs, int flags, fem_batch **batch) {
   if (!batch) return set_error(FEM_ERR_NULL_POINTER, "batch is NULL");
+  if (maxJobs < 1) return set_error(FEM_ERR_INVALID_ARGUMENT, "maxJobs %d < 1", maxJobs);
   fem_batch *b = new fem_batch();
+  b->maxJobs = maxJobs;
+  b->flags = flags;
   int dev = 0;
   (void)hipGetDevice(&dev);
   if (hipDeviceGetAttribute(&b->numCUs, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) b->numCUs = 0;
+  b->hCounts.assign(4 * fem_batch::kGroups, 0);
+  if (hipMalloc(reinterpret_cast<void **>(&b->dJobs), sizeof(MeshJob) * fem_batch::kGroups * maxJobs) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&b->dJobStart), sizeof(int) * fem_batch::kGroups * (maxJobs + 1)) !=
+          hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&b->dCounts), sizeof(int) * 4 * fem_batch::kGroups) != hipSuccess) {
+    free_batch(b);
+    delete b;
+    return set_error(FEM_ERR_OUT_OF_MEMORY, "hipMalloc of batch job tables failed");
+  }
   *batch = b;
   return FEM_OK;
 }
@@ -1159,17 +1217,20 @@ int fem_batch_add(fem_batch *batch, fem_plan *plan, const double *states, const 
   if (!batch || !plan) return set_error(FEM_ERR_NULL_POINTER, "batch/plan is NULL");
   if (!states || !nodeCoords) return set_error(FEM_ERR_NULL_POINTER, "states/nodeCoords is NULL");
   if (!residual && !matEntries) return set_error(FEM_ERR_NULL_POINTER, "residual and matEntries are both NULL");
-  if ((plan->flags & FEM_PLAN_RESIDUAL_ONLY) && matEntries) {
-    return set_error(FEM_ERR_INVALID_ARGUMENT, "plan was built residual-only; it cannot assemble the Jacobian");
-  }
   if ((plan->flags & FEM_PLAN_DETERMINISTIC) || (plan->flags & FEM_PLAN_RESIDUAL_ONLY)) {
     return set_error(FEM_ERR_UNSUPPORTED, "batches take plans built with default flags");
+  }
+  if ((batch->flags & FEM_BATCH_GRAPH) && !plan_streams(plan)) {
+    return set_error(FEM_ERR_UNSUPPORTED, "graph batches need plans whose parts are one batch per wavefront");
   }
   Material m;
   int rc = make_material(E, nu, t, m);
   if (rc) return rc;
   const int outputs = (residual && matEntries) ? 2 : (matEntries ? 1 : 0);
   fem_batch::Group &g = batch->groups[(plan->order - 1) * 3 + outputs];
+  if (int(g.jobs.size()) >= batch->maxJobs) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "batch group is full (%d jobs)", batch->maxJobs);
+  }
   g.jobs.push_back(make_job(plan, states, nodeCoords, residual, matEntries, m));
   g.jobStart.push_back(g.jobStart.back() + plan->numParts);
   g.stream = g.stream && plan_streams(plan);
@@ -1183,53 +1244,46 @@ int fem_batch_assemble(fem_batch *batch, int precision, void *stream, double *el
     return set_error(FEM_ERR_INVALID_ARGUMENT, "unknown precision mode %d", precision);
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
-  // Upload every non-empty group's job table before the timed launches.
-  for (auto &g : batch->groups) {
+  const bool useGraph = (batch->flags & FEM_BATCH_GRAPH) != 0;
+  // Job tables of non-empty groups, then the counts of all groups (asynchronous, before the launches).
+  for (int gi = 0; gi < fem_batch::kGroups; gi++) {
+    fem_batch::Group &g = batch->groups[gi];
+    batch->hCounts[4 * gi] = int(g.jobs.size());
+    batch->hCounts[4 * gi + 1] = g.jobStart.back();
     if (g.jobs.empty()) continue;
-    if (g.capJobs < g.jobs.size()) {
-      free_batch_group(g);
-      const size_t cap = std::max<size_t>(g.jobs.size(), 16);
-      if (hipMalloc(reinterpret_cast<void **>(&g.dJobs), cap * sizeof(MeshJob)) != hipSuccess ||
-          hipMalloc(reinterpret_cast<void **>(&g.dJobStart), (cap + 1) * sizeof(int)) != hipSuccess) {
-        free_batch_group(g);
-        return set_error(FEM_ERR_OUT_OF_MEMORY, "hipMalloc of batch job tables failed");
-      }
-      g.capJobs = cap;
-    }
-    FEM_HIP_CHECK(hipMemcpyAsync(g.dJobs, g.jobs.data(), g.jobs.size() * sizeof(MeshJob), hipMemcpyHostToDevice, s));
-    FEM_HIP_CHECK(hipMemcpyAsync(g.dJobStart, g.jobStart.data(), g.jobStart.size() * sizeof(int),
+    FEM_HIP_CHECK(hipMemcpyAsync(batch->dJobs + size_t(gi) * batch->maxJobs, g.jobs.data(),
+                                 g.jobs.size() * sizeof(MeshJob), hipMemcpyHostToDevice, s));
+    FEM_HIP_CHECK(hipMemcpyAsync(batch->dJobStart + size_t(gi) * (batch->maxJobs + 1), g.jobStart.data(),
+                                 g.jobStart.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  }
+  if (useGraph) {
+    FEM_HIP_CHECK(hipMemcpyAsync(batch->dCounts, batch->hCounts.data(), batch->hCounts.size() * sizeof(int),
                                  hipMemcpyHostToDevice, s));
+    if (!batch->graph[precision]) {
+      int rc = capture_batch_graph(batch, precision, s);
+      if (rc) return rc;
+    }
   }
   return timed(s, elapsedSeconds, [&]() -> int {
-    for (int gi = 0; gi < 12; gi++) {
+    if (useGraph) {
+      hipError_t e = hipGraphLaunch(batch->graph[precision], s);
+      if (e != hipSuccess) return set_error(FEM_ERR_HIP, "hipGraphLaunch failed: %s", hipGetErrorString(e));
+      return FEM_OK;
+    }
+    for (int gi = 0; gi < fem_batch::kGroups; gi++) {
       fem_batch::Group &g = batch->groups[gi];
       if (g.jobs.empty()) continue;
-      const int order = gi / 3 + 1, outputs = gi % 3;
-      OwnerLaunch L{};
-      int rc = make_basis(order, L.args.basis);
-      if (rc) return rc;
-      L.args.jobs = g.dJobs;
-      L.args.jobStart = g.dJobStart;
-      L.args.numJobs = int(g.jobs.size());
-      L.args.totalParts = g.jobStart.back();
-      L.precision = precision;
-      L.det = false;
-      L.res = outputs != 1;
-      L.jac = outputs != 0;
-      L.rOnlyParts = false;
-      L.stream = g.stream && batch->numCUs > 0;
-      L.streamGrid = stream_grid(batch->numCUs);
+      const OwnerLaunch L = batch_group_launch(batch, gi, precision, false);
       if (L.stream) {
-        launch_owner_any(order, L, s);
+        launch_owner_any(gi / 3 + 1, L, s);
       } else {
         // Parts too large for streaming: one per-part launch per job.
         for (size_t j = 0; j < g.jobs.size(); j++) {
           OwnerLaunch Lj = L;
-          Lj.args.jobs = g.dJobs + j;
-          Lj.args.jobStart = g.dJobStart;  // only [0] is read for a single job: first part 0 of the job
+          Lj.args.jobs = L.args.jobs + j;
           Lj.args.numJobs = 1;
           Lj.args.totalParts = g.jobs[j].numParts;
-          launch_owner_any(order, Lj, s);
+          launch_owner_any(gi / 3 + 1, Lj, s);
         }
       }
     }
@@ -1239,7 +1293,7 @@ int fem_batch_assemble(fem_batch *batch, int precision, void *stream, double *el
 
 int fem_batch_destroy(fem_batch *batch) {
   if (!batch) return FEM_OK;
-  for (auto &g : batch->groups) free_batch_group(g);
+  free_batch(batch);
   delete batch;
   return FEM_OK;
 }

@@ -1,0 +1,454 @@
+// Serving loop of libfem_amd: resident meshes, HBM-budget admission and continuous batching.
+//
+// Meshes are registered once (BSR pattern + resident plan + device coordinates). Clients submit
+// assembly requests (mesh, device states, residual-only or residual+Jacobian); the server allocates
+// each request's outputs when the request is admitted under the device-memory budget and keeps them
+// until the client releases them. A worker thread forms batches from the admitted requests and launches
+// each batch as fem_batch_assemble (one persistent launch per element order and output set, optionally
+// replayed through a captured hipGraph) on one of several streams ("slots"): requests that arrive while
+// earlier batches run are collected into the next batch, which starts on a free slot and overlaps the
+// running ones on the device.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "fem_internal.h"
+
+namespace {
+
+double now_seconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct ServerMesh {
+  fem_plan *plan = nullptr;
+  double *coords = nullptr;
+  int order = 0;
+  int numNodes = 0;
+  int64_t numBlocks = 0;
+  double E = 0, nu = 0, t = 0;
+  int64_t residentBytes = 0;
+};
+
+enum RequestState { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3, kReleased = 4 };
+
+struct Request {
+  int64_t id = 0;
+  int mesh = 0;
+  const double *states = nullptr;
+  bool withJacobian = false;
+  double *residual = nullptr;
+  double *values = nullptr;
+  int64_t bytes = 0;
+  RequestState state = kQueued;
+  int status = FEM_OK;
+  std::string error;
+  double tSubmit = 0, tLaunch = 0, tDone = 0;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  fem_batch *batch = nullptr;
+  std::vector<Request *> requests;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct fem_server {
+  fem_server_config cfg{};
+  int device = 0;
+  std::vector<ServerMesh> meshes;
+  int64_t residentBytes = 0;
+  int64_t outputBytes = 0;  // device bytes held by the output pool (in use + cached)
+  std::multimap<size_t, void *> freeBuffers;
+  std::unordered_map<int64_t, std::unique_ptr<Request>> requests;
+  std::deque<Request *> queue;
+  std::vector<Slot> slots;
+  int64_t nextId = 1;
+  int64_t batches = 0;
+  int64_t completed = 0;
+  std::mutex mu;
+  std::condition_variable cv;      // worker wake-up
+  std::condition_variable doneCv;  // request completion
+  std::thread worker;
+  bool stop = false;
+};
+
+namespace {
+
+int64_t output_bytes(const ServerMesh &m, bool withJacobian) {
+  return int64_t(m.numNodes) * 2 * int64_t(sizeof(double)) + (withJacobian ? m.numBlocks * 4 * 8 : 0);
+}
+
+// Pool allocation under the budget; frees cached buffers when that makes room. Caller holds mu.
+void *pool_alloc(fem_server *s, size_t bytes) {
+  auto it = s->freeBuffers.find(bytes);
+  if (it != s->freeBuffers.end()) {
+    void *p = it->second;
+    s->freeBuffers.erase(it);
+    return p;
+  }
+  while (s->residentBytes + s->outputBytes + int64_t(bytes) > s->cfg.hbmBudgetBytes && !s->freeBuffers.empty()) {
+    auto victim = s->freeBuffers.begin();
+    (void)hipFree(victim->second);
+    s->outputBytes -= int64_t(victim->first);
+    s->freeBuffers.erase(victim);
+  }
+  if (s->residentBytes + s->outputBytes + int64_t(bytes) > s->cfg.hbmBudgetBytes) return nullptr;
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  s->outputBytes += int64_t(bytes);
+  return p;
+}
+
+void pool_free(fem_server *s, void *p, size_t bytes) {
+  if (p) s->freeBuffers.emplace(bytes, p);
+}
+
+// Admits queued requests in FIFO order while their outputs fit the budget. Caller holds mu.
+void admit_into(fem_server *s, std::vector<Request *> &out, int capacity) {
+  int perGroup[12] = {0};
+  while (!s->queue.empty() && int(out.size()) < capacity) {
+    Request *r = s->queue.front();
+    const ServerMesh &m = s->meshes[size_t(r->mesh)];
+    const int g = (m.order - 1) * 3 + (r->withJacobian ? 2 : 0);
+    if (perGroup[g] >= s->cfg.maxBatchJobs) break;
+    const size_t rb = size_t(m.numNodes) * 2 * sizeof(double);
+    const size_t vb = r->withJacobian ? size_t(m.numBlocks) * 4 * sizeof(double) : 0;
+    double *res = static_cast<double *>(pool_alloc(s, rb));
+    if (!res) break;
+    double *vals = nullptr;
+    if (vb) {
+      vals = static_cast<double *>(pool_alloc(s, vb));
+      if (!vals) {
+        pool_free(s, res, rb);
+        break;
+      }
+    }
+    r->residual = res;
+    r->values = vals;
+    s->queue.pop_front();
+    perGroup[g]++;
+    out.push_back(r);
+  }
+}
+
+void complete_slot(fem_server *s, Slot &slot, int status, const char *err) {
+  const double t = now_seconds();
+  std::lock_guard<std::mutex> lk(s->mu);
+  for (Request *r : slot.requests) {
+    r->tDone = t;
+    r->status = status;
+    if (status != FEM_OK) {
+      r->state = kFailed;
+      r->error = err ? err : "";
+    } else {
+      r->state = kDone;
+    }
+    s->completed++;
+  }
+  slot.requests.clear();
+  slot.busy = false;
+  s->doneCv.notify_all();
+}
+
+void worker_loop(fem_server *s) {
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    // Retire finished batches.
+    for (Slot &slot : s->slots) {
+      if (!slot.busy) continue;
+      const hipError_t q = hipEventQuery(slot.done);
+      if (q == hipSuccess) {
+        complete_slot(s, slot, FEM_OK, nullptr);
+      } else if (q != hipErrorNotReady) {
+        complete_slot(s, slot, FEM_ERR_HIP, hipGetErrorString(q));
+      }
+    }
+    // Launch on free slots while requests wait.
+    bool launched = false;
+    for (Slot &slot : s->slots) {
+      if (slot.busy) continue;
+      std::vector<Request *> take;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop && s->queue.empty()) break;
+        admit_into(s, take, 12 * s->cfg.maxBatchJobs);
+        const double t = now_seconds();
+        for (Request *r : take) {
+          r->state = kRunning;
+          r->tLaunch = t;
+        }
+      }
+      if (take.empty()) break;
+      int rc = fem_batch_clear(slot.batch);
+      for (Request *r : take) {
+        if (rc) break;
+        const ServerMesh &m = s->meshes[size_t(r->mesh)];
+        rc = fem_batch_add(slot.batch, m.plan, r->states, m.coords, m.E, m.nu, m.t, r->residual,
+                           r->withJacobian ? r->values : nullptr);
+      }
+      if (!rc) rc = fem_batch_assemble(slot.batch, s->cfg.precision, slot.stream, nullptr);
+      if (!rc && hipEventRecord(slot.done, slot.stream) != hipSuccess) rc = FEM_ERR_HIP;
+      slot.requests = take;
+      slot.busy = true;
+      if (rc) {
+        complete_slot(s, slot, rc, fem_last_error());
+      } else {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->batches++;
+      }
+      launched = true;
+    }
+    bool anyBusy = false;
+    for (Slot &slot : s->slots) anyBusy = anyBusy || slot.busy;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      if (s->stop && s->queue.empty() && !anyBusy) return;
+      if (!launched) {
+        if (anyBusy) {
+          // Poll the device for completions at a fine grain.
+          s->cv.wait_for(lk, std::chrono::microseconds(20));
+        } else if (s->queue.empty()) {
+          s->cv.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+        } else {
+          // Requests wait for memory held by completed-but-unreleased results.
+          s->cv.wait_for(lk, std::chrono::microseconds(100));
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+using namespace fem;
+
+extern "C" {
+
+int fem_server_create(const fem_server_config *cfg, fem_server **server) {
+  if (!cfg || !server) return set_error(FEM_ERR_NULL_POINTER, "cfg/server is NULL");
+  if (cfg->hbmBudgetBytes <= 0 || cfg->maxBatchJobs < 1 || cfg->numSlots < 1 || cfg->numSlots > 8) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "budget > 0, maxBatchJobs >= 1 and 1 <= numSlots <= 8 required");
+  }
+  if (cfg->precision != FEM_PREC_FP64 && cfg->precision != FEM_PREC_MIXED) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "unknown precision mode %d", cfg->precision);
+  }
+  auto *s = new fem_server();
+  s->cfg = *cfg;
+  (void)hipGetDevice(&s->device);
+  s->slots.resize(size_t(cfg->numSlots));
+  for (Slot &slot : s->slots) {
+    if (hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&slot.done, hipEventDisableTiming) != hipSuccess ||
+        fem_batch_create(cfg->maxBatchJobs, cfg->useGraphs ? FEM_BATCH_GRAPH : 0, &slot.batch) != FEM_OK) {
+      fem_server_destroy(s);
+      return set_error(FEM_ERR_HIP, "creating server streams/events/batches failed");
+    }
+  }
+  s->worker = std::thread(worker_loop, s);
+  *server = s;
+  return FEM_OK;
+}
+
+int fem_server_add_mesh(fem_server *server, int order, int numNodes, int numElements, const int *connPtr,
+                        const int *conn, const double *nodeCoordsHost, double E, double nu, double t, int *meshId) {
+  if (!server || !meshId || !nodeCoordsHost) return set_error(FEM_ERR_NULL_POINTER, "NULL argument");
+  const int N = (order + 1) * (order + 1);
+  if (order < 1 || order > 4) return set_error(FEM_ERR_INVALID_ORDER, "element order %d outside 1..4", order);
+  std::vector<int> rowp(size_t(numNodes) + 1);
+  int rc = fem_bsr_pattern_rowp(numNodes, numElements, N, connPtr, conn, rowp.data());
+  if (rc) return rc;
+  std::vector<int> cols(size_t(rowp[size_t(numNodes)]));
+  rc = fem_bsr_pattern_cols(numNodes, numElements, N, connPtr, conn, rowp.data(), cols.data());
+  if (rc) return rc;
+  ServerMesh m;
+  m.order = order;
+  m.numNodes = numNodes;
+  m.numBlocks = rowp[size_t(numNodes)];
+  m.E = E;
+  m.nu = nu;
+  m.t = t;
+  rc = fem_plan_create(order, numNodes, numElements, connPtr, conn, rowp.data(), cols.data(), nodeCoordsHost, 0,
+                       &m.plan);
+  if (rc) return rc;
+  int64_t planBytes = 0;
+  fem_plan_info(m.plan, nullptr, nullptr, nullptr, &planBytes);
+  m.residentBytes = planBytes + int64_t(numNodes) * 16;
+  {
+    std::lock_guard<std::mutex> lk(server->mu);
+    if (server->residentBytes + server->outputBytes + m.residentBytes > server->cfg.hbmBudgetBytes) {
+      fem_plan_destroy(m.plan);
+      return set_error(FEM_ERR_BUDGET, "mesh needs %lld resident bytes; budget leaves %lld",
+                       (long long)m.residentBytes,
+                       (long long)(server->cfg.hbmBudgetBytes - server->residentBytes - server->outputBytes));
+    }
+    server->residentBytes += m.residentBytes;
+  }
+  if (hipMalloc(reinterpret_cast<void **>(&m.coords), size_t(numNodes) * 16) != hipSuccess ||
+      hipMemcpy(m.coords, nodeCoordsHost, size_t(numNodes) * 16, hipMemcpyHostToDevice) != hipSuccess) {
+    fem_plan_destroy(m.plan);
+    std::lock_guard<std::mutex> lk(server->mu);
+    server->residentBytes -= m.residentBytes;
+    return set_error(FEM_ERR_HIP, "uploading mesh coordinates failed");
+  }
+  std::lock_guard<std::mutex> lk(server->mu);
+  *meshId = int(server->meshes.size());
+  server->meshes.push_back(m);
+  return FEM_OK;
+}
+
+int fem_server_mesh_info(fem_server *server, int meshId, int64_t *numNodes, int64_t *numBlocks,
+                         int64_t *residentBytes) {
+  if (!server) return set_error(FEM_ERR_NULL_POINTER, "server is NULL");
+  std::lock_guard<std::mutex> lk(server->mu);
+  if (meshId < 0 || meshId >= int(server->meshes.size())) return set_error(FEM_ERR_INVALID_ARGUMENT, "bad mesh id");
+  const ServerMesh &m = server->meshes[size_t(meshId)];
+  if (numNodes) *numNodes = m.numNodes;
+  if (numBlocks) *numBlocks = m.numBlocks;
+  if (residentBytes) *residentBytes = m.residentBytes;
+  return FEM_OK;
+}
+
+int fem_server_submit(fem_server *server, int meshId, const double *states, int withJacobian, int64_t *requestId) {
+  if (!server || !states || !requestId) return set_error(FEM_ERR_NULL_POINTER, "NULL argument");
+  std::lock_guard<std::mutex> lk(server->mu);
+  if (meshId < 0 || meshId >= int(server->meshes.size())) return set_error(FEM_ERR_INVALID_ARGUMENT, "bad mesh id");
+  const ServerMesh &m = server->meshes[size_t(meshId)];
+  const int64_t need = output_bytes(m, withJacobian != 0);
+  if (server->residentBytes + need > server->cfg.hbmBudgetBytes) {
+    return set_error(FEM_ERR_BUDGET, "request outputs need %lld bytes; the budget leaves %lld after resident meshes",
+                     (long long)need, (long long)(server->cfg.hbmBudgetBytes - server->residentBytes));
+  }
+  auto r = std::make_unique<Request>();
+  r->id = server->nextId++;
+  r->mesh = meshId;
+  r->states = states;
+  r->withJacobian = withJacobian != 0;
+  r->bytes = need;
+  r->tSubmit = now_seconds();
+  *requestId = r->id;
+  server->queue.push_back(r.get());
+  server->requests.emplace(r->id, std::move(r));
+  server->cv.notify_one();
+  return FEM_OK;
+}
+
+int fem_server_poll(fem_server *server, int64_t requestId, int *done) {
+  if (!server || !done) return set_error(FEM_ERR_NULL_POINTER, "NULL argument");
+  std::lock_guard<std::mutex> lk(server->mu);
+  auto it = server->requests.find(requestId);
+  if (it == server->requests.end()) return set_error(FEM_ERR_INVALID_ARGUMENT, "unknown request %lld", (long long)requestId);
+  *done = it->second->state == kDone || it->second->state == kFailed;
+  return FEM_OK;
+}
+
+int fem_server_wait(fem_server *server, int64_t requestId, const double **residual, const double **values,
+                    double *times) {
+  if (!server) return set_error(FEM_ERR_NULL_POINTER, "server is NULL");
+  std::unique_lock<std::mutex> lk(server->mu);
+  auto it = server->requests.find(requestId);
+  if (it == server->requests.end()) return set_error(FEM_ERR_INVALID_ARGUMENT, "unknown request %lld", (long long)requestId);
+  Request *r = it->second.get();
+  server->doneCv.wait(lk, [&] { return r->state == kDone || r->state == kFailed || r->state == kReleased; });
+  if (r->state == kReleased) return set_error(FEM_ERR_INVALID_ARGUMENT, "request %lld was released", (long long)requestId);
+  if (residual) *residual = r->residual;
+  if (values) *values = r->withJacobian ? r->values : nullptr;
+  if (times) {
+    times[0] = r->tSubmit;
+    times[1] = r->tLaunch;
+    times[2] = r->tDone;
+  }
+  if (r->state == kFailed) return set_error(r->status, "request failed: %s", r->error.c_str());
+  return FEM_OK;
+}
+
+int fem_server_read(fem_server *server, int64_t requestId, double *residualHost, double *valuesHost) {
+  const double *res = nullptr, *vals = nullptr;
+  int rc = fem_server_wait(server, requestId, &res, &vals, nullptr);
+  if (rc) return rc;
+  int64_t nn = 0, nb = 0;
+  int mesh = 0;
+  {
+    std::lock_guard<std::mutex> lk(server->mu);
+    mesh = server->requests[requestId]->mesh;
+    nn = server->meshes[size_t(mesh)].numNodes;
+    nb = server->meshes[size_t(mesh)].numBlocks;
+  }
+  if (residualHost && hipMemcpy(residualHost, res, size_t(nn) * 16, hipMemcpyDeviceToHost) != hipSuccess) {
+    return set_error(FEM_ERR_HIP, "copying the residual failed");
+  }
+  if (valuesHost && vals && hipMemcpy(valuesHost, vals, size_t(nb) * 32, hipMemcpyDeviceToHost) != hipSuccess) {
+    return set_error(FEM_ERR_HIP, "copying the BSR values failed");
+  }
+  return FEM_OK;
+}
+
+int fem_server_release(fem_server *server, int64_t requestId) {
+  if (!server) return set_error(FEM_ERR_NULL_POINTER, "server is NULL");
+  std::lock_guard<std::mutex> lk(server->mu);
+  auto it = server->requests.find(requestId);
+  if (it == server->requests.end()) return set_error(FEM_ERR_INVALID_ARGUMENT, "unknown request %lld", (long long)requestId);
+  Request *r = it->second.get();
+  if (r->state != kDone && r->state != kFailed) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "request %lld has not completed", (long long)requestId);
+  }
+  const ServerMesh &m = server->meshes[size_t(r->mesh)];
+  pool_free(server, r->residual, size_t(m.numNodes) * 16);
+  if (r->values) pool_free(server, r->values, size_t(m.numBlocks) * 32);
+  server->requests.erase(it);
+  server->cv.notify_one();
+  return FEM_OK;
+}
+
+int fem_server_stats(fem_server *server, int64_t *batches, int64_t *completed, int64_t *bytesInUse) {
+  if (!server) return set_error(FEM_ERR_NULL_POINTER, "server is NULL");
+  std::lock_guard<std::mutex> lk(server->mu);
+  if (batches) *batches = server->batches;
+  if (completed) *completed = server->completed;
+  if (bytesInUse) *bytesInUse = server->residentBytes + server->outputBytes;
+  return FEM_OK;
+}
+
+int fem_server_destroy(fem_server *server) {
+  if (!server) return FEM_OK;
+  {
+    std::lock_guard<std::mutex> lk(server->mu);
+    server->stop = true;
+    server->cv.notify_all();
+  }
+  if (server->worker.joinable()) server->worker.join();
+  for (Slot &slot : server->slots) {
+    if (slot.stream) (void)hipStreamSynchronize(slot.stream);
+    if (slot.batch) fem_batch_destroy(slot.batch);
+    if (slot.done) (void)hipEventDestroy(slot.done);
+    if (slot.stream) (void)hipStreamDestroy(slot.stream);
+  }
+  for (auto &kv : server->requests) {
+    Request *r = kv.second.get();
+    if (r->residual) (void)hipFree(r->residual);
+    if (r->values) (void)hipFree(r->values);
+  }
+  for (auto &kv : server->freeBuffers) (void)hipFree(kv.second);
+  for (ServerMesh &m : server->meshes) {
+    fem_plan_destroy(m.plan);
+    (void)hipFree(m.coords);
+  }
+  delete server;
+  return FEM_OK;
+}
+
+}  // extern "C"

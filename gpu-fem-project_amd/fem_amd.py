@@ -24,6 +24,7 @@ FEM_PREC_FP64 = 0
 FEM_PREC_MIXED = 1
 FEM_PLAN_DETERMINISTIC = 1
 FEM_PLAN_RESIDUAL_ONLY = 2
+FEM_BATCH_GRAPH = 1
 
 STATUS_NAMES = {
     0: "FEM_OK",
@@ -71,11 +72,21 @@ SIGNATURES = {
     "fem_plan_assemble": ([_P, _P, _P, _D, _D, _D, _P, _P, _I, _P, ctypes.POINTER(_D)], _I),
     "fem_plan_destroy": ([_P], _I),
     "fem_bsr_spmv": ([_I, _P, _P, _P, _P, _P, _P], _I),
-    "fem_batch_create": ([ctypes.POINTER(_P)], _I),
+    "fem_batch_create": ([_I, _I, ctypes.POINTER(_P)], _I),
     "fem_batch_clear": ([_P], _I),
     "fem_batch_add": ([_P, _P, _P, _P, _D, _D, _D, _P, _P], _I),
     "fem_batch_assemble": ([_P, _I, _P, ctypes.POINTER(_D)], _I),
     "fem_batch_destroy": ([_P], _I),
+    "fem_server_create": ([_P, ctypes.POINTER(_P)], _I),
+    "fem_server_add_mesh": ([_P, _I, _I, _I, _P, _P, _P, _D, _D, _D, ctypes.POINTER(_I)], _I),
+    "fem_server_mesh_info": ([_P, _I, _PI64, _PI64, _PI64], _I),
+    "fem_server_submit": ([_P, _I, _P, _I, _PI64], _I),
+    "fem_server_poll": ([_P, _I64, ctypes.POINTER(_I)], _I),
+    "fem_server_wait": ([_P, _I64, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_D)], _I),
+    "fem_server_read": ([_P, _I64, _P, _P], _I),
+    "fem_server_release": ([_P, _I64], _I),
+    "fem_server_stats": ([_P, _PI64, _PI64, _PI64], _I),
+    "fem_server_destroy": ([_P], _I),
 }
 
 _lib = None
@@ -356,10 +367,10 @@ class Batch:
     """Many assemblies on resident plans in one call (fem_batch_*): one persistent launch per element
     order and output set."""
 
-    def __init__(self):
+    def __init__(self, max_jobs: int = 256, graph: bool = False):
         self._lib = load()
         handle = ctypes.c_void_p()
-        _check(self._lib.fem_batch_create(ctypes.byref(handle)))
+        _check(self._lib.fem_batch_create(max_jobs, FEM_BATCH_GRAPH if graph else 0, ctypes.byref(handle)))
         self.handle = handle
         self._keep = []
 
@@ -381,6 +392,79 @@ class Batch:
     def close(self):
         if self.handle:
             self._lib.fem_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbmBudgetBytes", ctypes.c_int64), ("maxBatchJobs", ctypes.c_int), ("numSlots", ctypes.c_int),
+                ("precision", ctypes.c_int), ("useGraphs", ctypes.c_int)]
+
+
+class Server:
+    """Serving loop (fem_server_*): resident meshes, budget admission, continuous batching."""
+
+    def __init__(self, budget_bytes: int, max_batch_jobs: int = 64, slots: int = 2, precision: int = FEM_PREC_MIXED,
+                 graphs: bool = True):
+        self._lib = load()
+        cfg = ServerConfig(int(budget_bytes), max_batch_jobs, slots, precision, 1 if graphs else 0)
+        handle = ctypes.c_void_p()
+        _check(self._lib.fem_server_create(ctypes.byref(cfg), ctypes.byref(handle)))
+        self.handle = handle
+        self.meshes = []
+
+    def add_mesh(self, mesh: Mesh, E: float, nu: float, t: float) -> int:
+        mid = ctypes.c_int()
+        cp = mesh.conn_ptr
+        _check(self._lib.fem_server_add_mesh(self.handle, mesh.order, mesh.num_nodes, mesh.num_elements, _ptr(cp),
+                                             _ptr(mesh.conn), _ptr(np.ascontiguousarray(mesh.coords)), E, nu, t,
+                                             ctypes.byref(mid)))
+        self.meshes.append(mesh)
+        return mid.value
+
+    def mesh_info(self, mesh_id: int):
+        v = [ctypes.c_int64() for _ in range(3)]
+        _check(self._lib.fem_server_mesh_info(self.handle, mesh_id, *[ctypes.byref(x) for x in v]))
+        return dict(zip(["nodes", "blocks", "resident_bytes"], [x.value for x in v]))
+
+    def submit(self, mesh_id: int, states, with_jacobian: bool) -> int:
+        rid = ctypes.c_int64()
+        _check(self._lib.fem_server_submit(self.handle, mesh_id, _dptr(states), 1 if with_jacobian else 0,
+                                           ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid: int):
+        times = (ctypes.c_double * 3)()
+        _check(self._lib.fem_server_wait(self.handle, rid, None, None, times))
+        return tuple(times)
+
+    def poll(self, rid: int) -> bool:
+        done = ctypes.c_int()
+        _check(self._lib.fem_server_poll(self.handle, rid, ctypes.byref(done)))
+        return bool(done.value)
+
+    def read(self, rid: int, num_nodes: int, num_blocks: int = 0):
+        res = np.zeros(2 * num_nodes)
+        vals = np.zeros(4 * num_blocks) if num_blocks else None
+        _check(self._lib.fem_server_read(self.handle, rid, _ptr(res), _ptr(vals)))
+        return res, vals
+
+    def release(self, rid: int):
+        _check(self._lib.fem_server_release(self.handle, rid))
+
+    def stats(self):
+        v = [ctypes.c_int64() for _ in range(3)]
+        _check(self._lib.fem_server_stats(self.handle, *[ctypes.byref(x) for x in v]))
+        return dict(zip(["batches", "completed", "bytes_in_use"], [x.value for x in v]))
+
+    def close(self):
+        if self.handle:
+            self._lib.fem_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -158,12 +158,52 @@ int fem_plan_destroy(fem_plan *plan);
  * flags 0. fem_batch_assemble uploads the job tables (asynchronously on `stream`) and launches; the
  * batch may be cleared and refilled once the call returns. */
 typedef struct fem_batch fem_batch;
-int fem_batch_create(fem_batch **batch);
+/* maxJobs: capacity per (order, outputs) group. flags: FEM_BATCH_GRAPH replays the twelve group
+ * launches as one captured hipGraph whose kernels read their job counts from device memory. */
+int fem_batch_create(int maxJobs, int flags, fem_batch **batch);
+#define FEM_BATCH_GRAPH 1
 int fem_batch_clear(fem_batch *batch);
 int fem_batch_add(fem_batch *batch, fem_plan *plan, const double *states, const double *nodeCoords, double E,
                   double nu, double t, double *residual, double *matEntries);
 int fem_batch_assemble(fem_batch *batch, int precision, void *stream, double *elapsedSeconds);
 int fem_batch_destroy(fem_batch *batch);
+
+/* ---- serving loop ------------------------------------------------------------------------------
+ * Resident meshes and continuous batching. A worker thread admits queued requests in FIFO order
+ * while their outputs fit the device-memory budget (resident meshes + live outputs), packs them into
+ * batches (fem_batch_*) and runs each batch on one of numSlots streams, so batches formed while
+ * others run overlap them on the device. Outputs are allocated by the server (pooled) when a request
+ * is admitted and held until fem_server_release. Request states are caller device memory that must be
+ * complete when submitted and stay valid until the request completes. Times are steady-clock seconds. */
+typedef struct fem_server fem_server;
+typedef struct fem_server_config {
+  int64_t hbmBudgetBytes; /* resident meshes + request outputs */
+  int maxBatchJobs;       /* jobs per (order, outputs) group in one batch */
+  int numSlots;           /* batches in flight (streams), 1..8 */
+  int precision;          /* FEM_PREC_FP64 or FEM_PREC_MIXED */
+  int useGraphs;          /* replay each batch's launches through a captured hipGraph */
+} fem_server_config;
+
+int fem_server_create(const fem_server_config *cfg, fem_server **server);
+/* Builds the mesh's BSR pattern and resident plan (host arrays in, as fem_plan_create) and uploads its
+ * coordinates. FEM_ERR_BUDGET if it does not fit. */
+int fem_server_add_mesh(fem_server *server, int order, int numNodes, int numElements, const int *connPtr,
+                        const int *conn, const double *nodeCoordsHost, double E, double nu, double t, int *meshId);
+int fem_server_mesh_info(fem_server *server, int meshId, int64_t *numNodes, int64_t *numBlocks,
+                         int64_t *residentBytes);
+/* Queues an assembly (withJacobian 0: residual only). FEM_ERR_BUDGET if its outputs can never fit. */
+int fem_server_submit(fem_server *server, int meshId, const double *states, int withJacobian, int64_t *requestId);
+int fem_server_poll(fem_server *server, int64_t requestId, int *done);
+/* Blocks until done; device pointers to the outputs (values NULL for residual-only) and times
+ * {submitted, launched, completed}. */
+int fem_server_wait(fem_server *server, int64_t requestId, const double **residual, const double **values,
+                    double *times);
+/* Waits, then copies the outputs to host memory (either pointer may be NULL). */
+int fem_server_read(fem_server *server, int64_t requestId, double *residualHost, double *valuesHost);
+/* Returns the request's outputs to the pool. */
+int fem_server_release(fem_server *server, int64_t requestId);
+int fem_server_stats(fem_server *server, int64_t *batches, int64_t *completed, int64_t *bytesInUse);
+int fem_server_destroy(fem_server *server);
 
 /* ---- BSR utilities (device) -------------------------------------------------------------------
  * y = A x for the 2x2-block BSR matrix (used to check R = K u). */
