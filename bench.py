@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import platform
 import sys
@@ -34,6 +35,15 @@ WORKLOADS = {
                                "1250 circumferential order-4 quads, 5x5 Gauss, smooth field + U(+-1e-5)"),
     "square64-p1": dict(kind="square", n=64, order=1, jitter=0.0, desc="unit square 64x64 order-1 quads, 2x2 Gauss, "
                         "u,v ~ U(+-1e-3) seed 1"),
+}
+POOL_WORKLOADS = {
+    "batch256-mixed": "256 independent meshes (parameter seed 6, mesh i seed 6e6+i): order U{1..4}, elements "
+                      "logU[1e3, 2e5], square or annulus 50/50, interior jitter +-20%, u,v ~ U(+-1e-3); mixed "
+                      "precision, one persistent launch per order/output group",
+    "serve-poisson": "serving: Poisson arrivals over a pool of 64 resident meshes (parameter seed 7): order "
+                     "U{1..4}, elements logU[1e3, 1e6], square or annulus; each request residual-only or "
+                     "residual+Jacobian with 50/50 odds; 200 GB HBM budget; mixed-precision served path vs "
+                     "plain FP64 per-request launches",
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
@@ -93,18 +103,368 @@ def load_traffic(workload, path_name):
     except (OSError, ValueError, KeyError):
         return None
 
+# ---- mesh pools: the batched and served workloads ------------------------------------------------------
+# Blocks per element of large meshes by order, and the reference's element->BSR map limit: its int32
+# entries hold block offsets times 4, so a mesh served by the plain (map-based) path has < 2^31 / 4 blocks.
+BLOCKS_PER_ELEMENT = {1: 9.1, 2: 64.1, 3: 225.5, 4: 577.0}
+MAP_BLOCK_LIMIT = (2 ** 31 - 1) // 4
+
+
+def mesh_pool(F, count, seed, e_max):
+    """count meshes drawn from SplitMix64(seed): order U{1..4}, elements logU[1e3, e_max], square (n = round
+    sqrt E) or annulus (n_r = round sqrt(E / 6.25), n_theta = round(E / n_r) made even) with 50/50 odds, interior
+    jitter +-20% of the smaller adjacent knot gap (mesh seed = seed * 1e6 + i). Element counts are capped so the
+    reference's int32 map can address every block."""
+    u = F.fill_uniform(4 * count, 0.0, 1.0, seed)
+    pool = []
+    for i in range(count):
+        order = 1 + min(3, int(4 * u[4 * i]))
+        ne = int(round(math.exp(math.log(1e3) + u[4 * i + 1] * (math.log(e_max) - math.log(1e3)))))
+        ne = min(ne, int(0.97 * MAP_BLOCK_LIMIT / BLOCKS_PER_ELEMENT[order]))
+        mseed = seed * 10 ** 6 + i
+        if u[4 * i + 2] < 0.5:
+            n = max(1, round(math.sqrt(ne)))
+            mesh = F.square_mesh(n, order, jitter=0.2, seed=mseed)
+        else:
+            nr = max(1, round(math.sqrt(ne / 6.25)))
+            nt = max(4, round(ne / nr))
+            nt += nt % 2
+            mesh = F.annulus_mesh(nr, nt, order, jitter=0.2, seed=mseed)
+        pool.append(mesh)
+    return pool
+
+
+def rel_err(torch, a, b):
+    return float((a - b).abs().max() / b.abs().max().clamp_min(1e-300))
+
+
+def bench_batch(args, F, torch):
+    """256 meshes assembled by one batch call (one persistent launch per order/output group, graph replay)
+    against one plan launch per mesh and against plain FP64 per-mesh atomic launches."""
+    dev = torch.device("cuda:0")
+    count = args.pool or 256
+    prec = F.FEM_PREC_FP64 if args.precision == "fp64" else F.FEM_PREC_MIXED
+    log(f"building {count} meshes")
+    meshes = mesh_pool(F, count, 6, 2e5)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    items = []
+    for i, m in enumerate(meshes):
+        pat = F.bsr_pattern(m)
+        u = F.fill_uniform(2 * m.num_nodes, -1e-3, 1e-3, 6 * 10 ** 6 + i)
+        it = dict(mesh=m, blocks=pat.num_blocks, plan=F.Plan(m, pat), coords=d(m.coords), states=d(u),
+                  res=torch.zeros(2 * m.num_nodes, dtype=torch.float64, device=dev),
+                  vals=torch.zeros(4 * pat.num_blocks, dtype=torch.float64, device=dev),
+                  conn=d(m.conn), conn_ptr=d(m.conn_ptr), map=d(F.element_bcsr_map(m, pat)))
+        items.append(it)
+        if (i + 1) % 32 == 0:
+            log(f"  {i + 1} meshes ready")
+    total_el = sum(m.num_elements for m in meshes)
+    b_alg = sum(algorithmic_bytes(it["mesh"], it["blocks"]) for it in items)
+    by_order = {p: sum(1 for m in meshes if m.order == p) for p in range(1, 5)}
+    log(f"{total_el} elements in total, orders {by_order}")
+    stream = torch.cuda.current_stream()
+    graph = True
+    batch = F.Batch(max_jobs=count, graph=True)
+    try:
+        for it in items:
+            batch.add(it["plan"], it["states"], it["coords"], E_MOD, NU, THICK, res=it["res"], values=it["vals"])
+    except F.FemError as e:
+        log(f"graph batch unavailable ({e}); using launches")
+        batch.close()
+        graph = False
+        batch = F.Batch(max_jobs=count, graph=False)
+        for it in items:
+            batch.add(it["plan"], it["states"], it["coords"], E_MOD, NU, THICK, res=it["res"], values=it["vals"])
+
+    def time_steps(step, steps):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    t_batch = time_steps(lambda: batch.assemble(precision=prec, stream=stream), args.steps)
+    log(f"batch: {t_batch * 1e3:.3f} ms per assembly of all {count} meshes")
+
+    def per_mesh_plans():
+        for it in items:
+            it["plan"].assemble(it["states"], it["coords"], E_MOD, NU, THICK, res=it["res"], values=it["vals"],
+                                precision=prec, stream=stream)
+
+    t_plans = time_steps(per_mesh_plans, max(1, args.steps // 2))
+    log(f"one plan launch per mesh: {t_plans * 1e3:.3f} ms")
+
+    def per_mesh_plain():
+        for it in items:
+            m = it["mesh"]
+            it["res"].zero_()
+            it["vals"].zero_()
+            F.jacobian(m.order, it["conn_ptr"], it["conn"], m.num_elements, it["states"], it["coords"], E_MOD, NU,
+                       THICK, it["map"], it["res"], it["vals"], precision=F.FEM_PREC_FP64, stream=stream)
+
+    t_plain = time_steps(per_mesh_plain, max(1, args.steps // 5))
+    log(f"plain FP64 atomic launch per mesh: {t_plain * 1e3:.3f} ms")
+
+    check = None
+    if not args.no_check:
+        # Every mesh: the batch result against the plain FP64 result (the element-parallel atomic path).
+        batch.assemble(precision=prec, stream=stream)
+        torch.cuda.synchronize()
+        got = [(it["res"].clone(), it["vals"].clone()) for it in items]
+        per_mesh_plain()
+        torch.cuda.synchronize()
+        er = max(rel_err(torch, g[0], it["res"]) for g, it in zip(got, items))
+        ev = max(rel_err(torch, g[1], it["vals"]) for g, it in zip(got, items))
+        check = {"max_rel_err_residual_vs_fp64": er, "max_rel_err_values_vs_fp64": ev}
+        log(f"checks: {check}")
+        del got
+    value = total_el / t_batch
+    out = {
+        "metric": "batched res+Jac elements/s",
+        "value": value,
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_batch * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == F.FEM_PREC_FP64 else "f32+f64acc",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {POOL_WORKLOADS[args.workload]}", "meshes": count,
+                   "elements": total_el, "orders": by_order, "graph": graph, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": b_alg / t_batch / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": b_alg / t_batch / 1e9 / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": b_alg},
+        "baselines": {"one_plan_launch_per_mesh_ms": t_plans * 1e3, "plain_fp64_atomic_per_mesh_ms": t_plain * 1e3,
+                      "speedup_vs_plain": t_plain / t_batch},
+        "cpu_baseline": None,
+        "checks": check,
+    }
+    print(json.dumps(out), flush=True)
+    batch.close()
+    for it in items:
+        it["plan"].close()
+
+
+def request_stream(F, n, seed, pool_size):
+    """Requests drawn from SplitMix64(seed): mesh U{0..pool-1}, residual+Jacobian with odds 1/2, and a unit-rate
+    exponential inter-arrival gap (scaled by 1/lambda by the caller)."""
+    u = F.fill_uniform(3 * n, 0.0, 1.0, seed)
+    mesh = np.minimum((u[0::3] * pool_size).astype(np.int64), pool_size - 1)
+    jac = u[1::3] < 0.5
+    gaps = -np.log1p(-u[2::3])
+    return mesh, jac, np.cumsum(gaps)
+
+
+def wait_until(t):
+    while True:
+        d = t - time.monotonic()
+        if d <= 0:
+            return
+        if d > 2e-3:
+            time.sleep(d - 1e-3)
+
+
+def bench_serve(args, F, torch):
+    """Poisson arrivals over 64 resident meshes: the serving loop (mixed precision, continuous batching, graph
+    replay) against plain FP64 per-request launches (FIFO, one request at a time, residual and Jacobian as
+    separate element-parallel atomic launches). Latency = completion - scheduled arrival."""
+    import threading
+    import queue
+
+    dev = torch.device("cuda:0")
+    pool_size = args.pool or 64
+    log(f"building {pool_size} meshes")
+    meshes = mesh_pool(F, pool_size, 7, 1e6)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    srv = F.Server(budget_bytes=200 * 10 ** 9, max_batch_jobs=64, slots=3, precision=F.FEM_PREC_MIXED,
+                   graphs=True)
+    plain = []
+    max_nodes = max_blocks = 0
+    for i, m in enumerate(meshes):
+        pat = F.bsr_pattern(m)
+        mid = srv.add_mesh(m, E_MOD, NU, THICK)
+        g = torch.Generator(device=dev)
+        g.manual_seed(7 * 10 ** 6 + i)
+        states = [torch.empty(2 * m.num_nodes, dtype=torch.float64, device=dev).uniform_(-1e-3, 1e-3, generator=g)
+                  for _ in range(2)]
+        plain.append(dict(mesh=m, mid=mid, blocks=pat.num_blocks, coords=d(m.coords), conn=d(m.conn),
+                          conn_ptr=d(m.conn_ptr), map=d(F.element_bcsr_map(m, pat)), states=states))
+        max_nodes = max(max_nodes, m.num_nodes)
+        max_blocks = max(max_blocks, pat.num_blocks)
+        if (i + 1) % 8 == 0:
+            log(f"  {i + 1} meshes resident ({srv.stats()['bytes_in_use'] / 1e9:.1f} GB in the server)")
+    orders = {p: sum(1 for m in meshes if m.order == p) for p in range(1, 5)}
+    sizes = sorted(m.num_elements for m in meshes)
+    log(f"pool: orders {orders}, elements min {sizes[0]} median {sizes[len(sizes) // 2]} max {sizes[-1]}")
+    res_buf = torch.zeros(2 * max_nodes, dtype=torch.float64, device=dev)
+    val_buf = torch.zeros(4 * max_blocks, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+
+    def plain_one(k, jac, state):
+        p = plain[k]
+        m = p["mesh"]
+        res = res_buf[:2 * m.num_nodes]
+        res.zero_()
+        F.residual(m.order, p["conn_ptr"], p["conn"], m.num_elements, p["states"][state], p["coords"], E_MOD, NU,
+                   THICK, res, stream=stream)
+        if jac:
+            vals = val_buf[:4 * p["blocks"]]
+            vals.zero_()
+            F.jacobian(m.order, p["conn_ptr"], p["conn"], m.num_elements, p["states"][state], p["coords"], E_MOD,
+                       NU, THICK, p["map"], None, vals, stream=stream)
+        stream.synchronize()
+
+    def run_plain(mesh_ids, jacs, arrivals):
+        n = len(mesh_ids)
+        lat = np.zeros(n)
+        t0 = time.monotonic() + 0.05
+        for i in range(n):
+            target = t0 + arrivals[i]
+            wait_until(target)
+            plain_one(int(mesh_ids[i]), bool(jacs[i]), i % 2)
+            lat[i] = time.monotonic() - target
+            if (i + 1) % 2000 == 0:
+                log(f"    plain {i + 1}/{n}")
+        return lat, time.monotonic() - t0
+
+    def run_served(mesh_ids, jacs, arrivals):
+        n = len(mesh_ids)
+        lat = np.zeros(n)
+        q = queue.Queue()
+        t0 = time.monotonic() + 0.05
+        errors = []
+
+        def reaper():
+            try:
+                for i in range(n):
+                    rid = q.get()
+                    times = srv.wait(rid)
+                    lat[i] = times[2] - (t0 + arrivals[i])
+                    srv.release(rid)
+            except Exception as e:  # reported below
+                errors.append(e)
+
+        th = threading.Thread(target=reaper)
+        th.start()
+        for i in range(n):
+            wait_until(t0 + arrivals[i])
+            q.put(srv.submit(plain[int(mesh_ids[i])]["mid"], plain[int(mesh_ids[i])]["states"][i % 2],
+                             bool(jacs[i])))
+            if (i + 1) % 2000 == 0:
+                log(f"    served {i + 1}/{n}")
+        th.join()
+        if errors:
+            raise errors[0]
+        return lat, time.monotonic() - t0
+
+    n_main = args.requests
+    mesh_ids, jacs, unit_arrivals = request_stream(F, max(n_main, args.side_requests), 7, pool_size)
+    elems = np.array([m.num_elements for m in meshes])
+
+    # Warm-up: every mesh once each way on both paths (graph capture, output-pool fill).
+    for k in range(pool_size):
+        for jac in (False, True):
+            plain_one(k, jac, 0)
+    warm = np.zeros(2 * pool_size)
+    run_served(np.repeat(np.arange(pool_size), 2), np.tile([False, True], pool_size), warm)
+
+    # Saturated throughput: the first S requests all present at t = 0.
+    sat = min(n_main, 1000)
+    zero = np.zeros(sat)
+    _, t_plain_sat = run_plain(mesh_ids[:sat], jacs[:sat], zero)
+    _, t_served_sat = run_served(mesh_ids[:sat], jacs[:sat], zero)
+    sat_el = float(elems[mesh_ids[:sat]].sum())
+    plain_rate = sat / t_plain_sat
+    log(f"saturated: plain {plain_rate:.1f} req/s, served {sat / t_served_sat:.1f} req/s")
+
+    loads = {}
+    for load, n in ((0.8, n_main), (0.5, args.side_requests), (0.95, args.side_requests)):
+        lam = load * plain_rate
+        arr = unit_arrivals[:n] / lam
+        log(f"load {load}: {n} requests at {lam:.1f} req/s")
+        lp, tp = run_plain(mesh_ids[:n], jacs[:n], arr)
+        ls, ts = run_served(mesh_ids[:n], jacs[:n], arr)
+        el = float(elems[mesh_ids[:n]].sum())
+        loads[str(load)] = {
+            "requests": n, "arrival_rate_per_s": lam,
+            "plain": {"p50_ms": float(np.percentile(lp, 50) * 1e3), "p99_ms": float(np.percentile(lp, 99) * 1e3),
+                      "throughput_req_s": n / tp, "elements_per_s": el / tp},
+            "served": {"p50_ms": float(np.percentile(ls, 50) * 1e3), "p99_ms": float(np.percentile(ls, 99) * 1e3),
+                       "throughput_req_s": n / ts, "elements_per_s": el / ts},
+        }
+        log(f"  {loads[str(load)]}")
+
+    check = None
+    if not args.no_check:
+        # Served (mixed) against plain FP64 for one residual+Jacobian request on each mesh.
+        er = ev = 0.0
+        for k, p in enumerate(plain):
+            rid = srv.submit(p["mid"], p["states"][0], True)
+            r_host, v_host = srv.read(rid, p["mesh"].num_nodes, p["blocks"])
+            srv.release(rid)
+            plain_one(k, True, 0)
+            r_ref = res_buf[:2 * p["mesh"].num_nodes]
+            v_ref = val_buf[:4 * p["blocks"]]
+            er = max(er, rel_err(torch, torch.from_numpy(r_host).to(dev), r_ref))
+            ev = max(ev, rel_err(torch, torch.from_numpy(v_host).to(dev), v_ref))
+        check = {"max_rel_err_residual_vs_fp64": er, "max_rel_err_values_vs_fp64": ev}
+        log(f"checks: {check}")
+    st = srv.stats()
+    main_load = loads["0.8"]
+    out = {
+        "metric": "served p50/p99 latency vs plain",
+        "value": main_load["served"]["p99_ms"],
+        "unit": "ms (p99 latency at 0.8x the plain path's saturated rate)",
+        "n_gpus": 1,
+        "steps": n_main,
+        "warmup": 2 * pool_size,
+        "ms_per_step": main_load["served"]["p50_ms"],
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64acc",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {POOL_WORKLOADS[args.workload]}", "pool": pool_size,
+                   "orders": orders, "elements_min_median_max": [sizes[0], sizes[len(sizes) // 2], sizes[-1]],
+                   "slots": 3, "max_batch_jobs": 64, "budget_bytes": 200 * 10 ** 9, "parallelism": "single GPU"},
+        "saturated": {"requests": sat, "plain_req_s": plain_rate, "served_req_s": sat / t_served_sat,
+                      "plain_elements_s": sat_el / t_plain_sat, "served_elements_s": sat_el / t_served_sat},
+        "loads": loads,
+        "server": st,
+        "cpu_baseline": None,
+        "checks": check,
+    }
+    print(json.dumps(out), flush=True)
+    srv.close()
+
+
+def run_pool_workload(args, F, torch):
+    if args.workload == "batch256-mixed":
+        bench_batch(args, F, torch)
+    else:
+        bench_serve(args, F, torch)
+
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="square1000-p2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="square1000-p2", choices=sorted(WORKLOADS) + sorted(POOL_WORKLOADS))
     ap.add_argument("--path", default="plan", choices=["plan", "plan-det", "atomic"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--requests", type=int, default=10000, help="serve-poisson: requests at the main load")
+    ap.add_argument("--side-requests", type=int, default=2000, help="serve-poisson: requests at 0.5 and 0.95 load")
+    ap.add_argument("--pool", type=int, default=0, help="override the mesh-pool size (0: the workload's own)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X, and the assembly path is sized for "
@@ -116,6 +476,9 @@ def main():
     import fem_amd as F
 
     F.load()
+    if args.workload in POOL_WORKLOADS:
+        run_pool_workload(args, F, torch)
+        return
     dev = torch.device("cuda:0")
     prec = F.FEM_PREC_FP64 if args.precision == "fp64" else F.FEM_PREC_MIXED
     log(f"building workload {args.workload}")

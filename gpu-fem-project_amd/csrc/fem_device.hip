@@ -1120,12 +1120,22 @@ struct fem_batch {
   int flags = 0;
   int numCUs = 0;
   int numJobs = 0;
-  // Device tables at fixed addresses (a captured graph refers to them).
-  MeshJob *dJobs = nullptr;    // [kGroups][maxJobs]
-  int *dJobStart = nullptr;    // [kGroups][maxJobs + 1]
-  int *dCounts = nullptr;      // [kGroups][4]: numJobs, totalParts
-  std::vector<int> hCounts;    // host image of dCounts
+  // One device block at a fixed address (a captured graph refers to it), laid out as
+  //   counts [kGroups][4] int (numJobs, totalParts) | jobStart [kGroups][maxJobs + 1] int | jobs [kGroups][maxJobs]
+  // and mirrored in pinned host memory, so each assembly stages its tables with one asynchronous copy.
+  unsigned char *dTables = nullptr;
+  unsigned char *hTables = nullptr;
+  size_t jobStartOffset = 0, jobsOffset = 0, tableBytes = 0;
+  hipEvent_t staged = nullptr;  // recorded after the copy: the host mirror is rewritten only once it has run
+  bool stagedPending = false;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // per precision
+
+  int *dCounts() const { return reinterpret_cast<int *>(dTables); }
+  int *dJobStart() const { return reinterpret_cast<int *>(dTables + jobStartOffset); }
+  MeshJob *dJobs() const { return reinterpret_cast<MeshJob *>(dTables + jobsOffset); }
+  int *hCounts() const { return reinterpret_cast<int *>(hTables); }
+  int *hJobStart() const { return reinterpret_cast<int *>(hTables + jobStartOffset); }
+  MeshJob *hJobs() const { return reinterpret_cast<MeshJob *>(hTables + jobsOffset); }
 };
 
 namespace {
@@ -1135,18 +1145,21 @@ void free_batch(fem_batch *b) {
     if (g) (void)hipGraphExecDestroy(g);
     g = nullptr;
   }
-  if (b->dJobs) (void)hipFree(b->dJobs);
-  if (b->dJobStart) (void)hipFree(b->dJobStart);
-  if (b->dCounts) (void)hipFree(b->dCounts);
+  if (b->staged) {
+    if (b->stagedPending) (void)hipEventSynchronize(b->staged);
+    (void)hipEventDestroy(b->staged);
+  }
+  if (b->dTables) (void)hipFree(b->dTables);
+  if (b->hTables) (void)hipHostFree(b->hTables);
 }
 
 OwnerLaunch batch_group_launch(const fem_batch *b, int gi, int precision, bool fromDevice) {
   const fem_batch::Group &g = b->groups[gi];
   OwnerLaunch L{};
   make_basis(gi / 3 + 1, L.args.basis);
-  L.args.jobs = b->dJobs + size_t(gi) * b->maxJobs;
-  L.args.jobStart = b->dJobStart + size_t(gi) * (b->maxJobs + 1);
-  L.args.counts = fromDevice ? b->dCounts + 4 * gi : nullptr;
+  L.args.jobs = b->dJobs() + size_t(gi) * b->maxJobs;
+  L.args.jobStart = b->dJobStart() + size_t(gi) * (b->maxJobs + 1);
+  L.args.counts = fromDevice ? b->dCounts() + 4 * gi : nullptr;
   L.args.numJobs = int(g.jobs.size());
   L.args.totalParts = g.jobStart.back();
   L.precision = precision;
@@ -1188,14 +1201,22 @@ int fem_batch_create(int maxJobs, int flags, fem_batch **batch) {
   int dev = 0;
   (void)hipGetDevice(&dev);
   if (hipDeviceGetAttribute(&b->numCUs, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) b->numCUs = 0;
-  b->hCounts.assign(4 * fem_batch::kGroups, 0);
-  if (hipMalloc(reinterpret_cast<void **>(&b->dJobs), sizeof(MeshJob) * fem_batch::kGroups * maxJobs) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void **>(&b->dJobStart), sizeof(int) * fem_batch::kGroups * (maxJobs + 1)) !=
-          hipSuccess ||
-      hipMalloc(reinterpret_cast<void **>(&b->dCounts), sizeof(int) * 4 * fem_batch::kGroups) != hipSuccess) {
+  const size_t G = fem_batch::kGroups;
+  b->jobStartOffset = 4 * G * sizeof(int);
+  b->jobsOffset = (b->jobStartOffset + G * size_t(maxJobs + 1) * sizeof(int) + 255) / 256 * 256;
+  b->tableBytes = b->jobsOffset + G * size_t(maxJobs) * sizeof(MeshJob);
+  if (hipMalloc(reinterpret_cast<void **>(&b->dTables), b->tableBytes) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&b->hTables), b->tableBytes, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&b->staged, hipEventDisableTiming) != hipSuccess) {
     free_batch(b);
     delete b;
-    return set_error(FEM_ERR_OUT_OF_MEMORY, "hipMalloc of batch job tables failed");
+    return set_error(FEM_ERR_OUT_OF_MEMORY, "allocating batch job tables (%zu bytes) failed", b->tableBytes);
+  }
+  std::memset(b->hTables, 0, b->tableBytes);
+  if (hipMemcpy(b->dTables, b->hTables, b->tableBytes, hipMemcpyHostToDevice) != hipSuccess) {
+    free_batch(b);
+    delete b;
+    return set_error(FEM_ERR_HIP, "initialising batch job tables failed");
   }
   *batch = b;
   return FEM_OK;
@@ -1245,24 +1266,30 @@ int fem_batch_assemble(fem_batch *batch, int precision, void *stream, double *el
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
   const bool useGraph = (batch->flags & FEM_BATCH_GRAPH) != 0;
-  // Job tables of non-empty groups, then the counts of all groups (asynchronous, before the launches).
+  // The previous call's copy must have run before the pinned mirror is rewritten.
+  if (batch->stagedPending) {
+    FEM_HIP_CHECK(hipEventSynchronize(batch->staged));
+    batch->stagedPending = false;
+  }
+  // Counts of all groups and job tables of non-empty groups, staged in the mirror and copied up to the
+  // end of the last non-empty group's jobs.
+  size_t copyEnd = batch->jobsOffset;
   for (int gi = 0; gi < fem_batch::kGroups; gi++) {
     fem_batch::Group &g = batch->groups[gi];
-    batch->hCounts[4 * gi] = int(g.jobs.size());
-    batch->hCounts[4 * gi + 1] = g.jobStart.back();
+    batch->hCounts()[4 * gi] = int(g.jobs.size());
+    batch->hCounts()[4 * gi + 1] = g.jobStart.back();
     if (g.jobs.empty()) continue;
-    FEM_HIP_CHECK(hipMemcpyAsync(batch->dJobs + size_t(gi) * batch->maxJobs, g.jobs.data(),
-                                 g.jobs.size() * sizeof(MeshJob), hipMemcpyHostToDevice, s));
-    FEM_HIP_CHECK(hipMemcpyAsync(batch->dJobStart + size_t(gi) * (batch->maxJobs + 1), g.jobStart.data(),
-                                 g.jobStart.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    std::memcpy(batch->hJobs() + size_t(gi) * batch->maxJobs, g.jobs.data(), g.jobs.size() * sizeof(MeshJob));
+    std::memcpy(batch->hJobStart() + size_t(gi) * (batch->maxJobs + 1), g.jobStart.data(),
+                g.jobStart.size() * sizeof(int));
+    copyEnd = batch->jobsOffset + (size_t(gi) * batch->maxJobs + g.jobs.size()) * sizeof(MeshJob);
   }
-  if (useGraph) {
-    FEM_HIP_CHECK(hipMemcpyAsync(batch->dCounts, batch->hCounts.data(), batch->hCounts.size() * sizeof(int),
-                                 hipMemcpyHostToDevice, s));
-    if (!batch->graph[precision]) {
-      int rc = capture_batch_graph(batch, precision, s);
-      if (rc) return rc;
-    }
+  FEM_HIP_CHECK(hipMemcpyAsync(batch->dTables, batch->hTables, copyEnd, hipMemcpyHostToDevice, s));
+  FEM_HIP_CHECK(hipEventRecord(batch->staged, s));
+  batch->stagedPending = true;
+  if (useGraph && !batch->graph[precision]) {
+    int rc = capture_batch_graph(batch, precision, s);
+    if (rc) return rc;
   }
   return timed(s, elapsedSeconds, [&]() -> int {
     if (useGraph) {
