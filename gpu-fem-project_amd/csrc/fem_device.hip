@@ -69,6 +69,37 @@ struct Material {
   double mu, lambda, t;
 };
 
+// Pointers read from device tables are generic to the compiler, which then issues flat loads; those also
+// count against the LDS wait counter and would stall LDS waits on outstanding HBM loads. Data reached
+// through these casts is loaded with global (vector-memory-only) instructions.
+// (The host pass of the compiler parses device functions too; address spaces exist only in the device pass.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FEM_GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define FEM_GLOBAL_AS
+#endif
+template <typename T>
+using gptr = const FEM_GLOBAL_AS T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> gmem(const T *p) {
+  return (gptr<T>)p;
+}
+
+template <typename T>
+using gptr_mut = FEM_GLOBAL_AS T *;
+template <typename T>
+__device__ __forceinline__ gptr_mut<T> gmem_mut(T *p) {
+  return (gptr_mut<T>)p;
+}
+
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS operations, then s_barrier. Unlike
+// __syncthreads() it does not wait for outstanding global loads and stores, so prefetches stay in flight
+// across it. The memory clobber keeps the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Wave-uniform value (keeps part- and job-level indices in scalar registers).
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 template <typename T>
 struct Vec2;
 template <>
@@ -369,7 +400,7 @@ struct PartDesc {
 };
 
 __device__ __forceinline__ PartDesc part_desc(const MeshJob &J, int p) {
-  const int4 a = J.partDesc[2 * p], b = J.partDesc[2 * p + 1];
+  const int4 a = gmem(J.partDesc)[2 * p], b = gmem(J.partDesc)[2 * p + 1];
   return PartDesc{a.x, a.y, a.z, a.w, b.x, b.y, b.z};
 }
 
@@ -388,18 +419,18 @@ struct Gather {
 // Level 1: node id and element id of visit `visit` (always in bounds: callers clamp the visit).
 template <int P>
 __device__ __forceinline__ void gather_level1(const MeshJob &J, int visit, int idx, Gather<P> &g) {
-  g.node = J.visitConn[int64_t(visit) * Ord<P>::N + idx];
-  g.eid = J.visitEid[visit];
+  g.node = gmem(J.visitConn)[int64_t(visit) * Ord<P>::N + idx];
+  g.eid = gmem(J.visitEid)[visit];
 }
 
 // Level 2: coordinates, states, owner row and column ranks of the node/element of level 1.
 template <int P>
 __device__ __forceinline__ void gather_level2(const MeshJob &J, int idx, Gather<P> &g) {
-  g.xy = *reinterpret_cast<const double2 *>(J.coords + 2 * int64_t(g.node));
-  g.uv = *reinterpret_cast<const double2 *>(J.states + 2 * int64_t(g.node));
-  g.lrow = J.nodeLrow[g.node];
+  g.xy = gmem(reinterpret_cast<const double2 *>(J.coords))[g.node];
+  g.uv = gmem(reinterpret_cast<const double2 *>(J.states))[g.node];
+  g.lrow = gmem(J.nodeLrow)[g.node];
   constexpr int W = Ord<P>::RPAD / 4;
-  const uint32_t *r = J.rank + (int64_t(g.eid) * Ord<P>::N + idx) * W;
+  const gptr<uint32_t> r = gmem(J.rank) + (int64_t(g.eid) * Ord<P>::N + idx) * W;
 #pragma unroll
   for (int i = 0; i < W; i++) g.rk[i] = r[i];
 }
@@ -451,17 +482,17 @@ __device__ __forceinline__ void owner_write(const MeshJob &J, const double *acc,
                                             const int4 *lruns, int nruns, const int *lrows, int nr) {
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   if (JAC && J.values) {
+    const gptr_mut<double2> values = gmem_mut(reinterpret_cast<double2 *>(J.values));
     for (int r = wave; r < nruns; r += WAVES) {
       const int4 run = lruns[r];
       const double2 *src = reinterpret_cast<const double2 *>(acc) + 2 * run.x;
-      double2 *dst = reinterpret_cast<double2 *>(J.values) + 2 * int64_t(run.y);
+      const gptr_mut<double2> dst = values + 2 * int64_t(run.y);
       for (int i = lane; i < 2 * run.z; i += 64) dst[i] = src[i];
     }
   }
   if (RES && J.residual) {
-    for (int i = threadIdx.x; i < nr; i += blockDim.x) {
-      reinterpret_cast<double2 *>(J.residual)[lrows[i]] = reinterpret_cast<const double2 *>(racc)[i];
-    }
+    const gptr_mut<double2> res = gmem_mut(reinterpret_cast<double2 *>(J.residual));
+    for (int i = int(threadIdx.x); i < nr; i += WAVES * 64) res[lrows[i]] = reinterpret_cast<const double2 *>(racc)[i];
   }
 }
 
@@ -544,10 +575,7 @@ __global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args)
   owner_write<RES, JAC, WAVES>(J, acc, racc, lruns, d.nruns, lrows, d.nr);
 }
 
-// Streaming kernel for parts that take one batch per wavefront (the default partition), over one or
-// many jobs (a batch of resident meshes of one order). A persistent workgroup walks a contiguous range
-// of global parts; while part k is computed it loads part k+1's row tables and element gathers and
-// part k+2's descriptor. Only the accumulator set-up and the write-out separate two parts.
+// One thread's entries of a part's row tables: owned row `tid` (node, LDS block base) and run `tid`.
 struct PartTables {
   int row, lbase;
   int runLds, runGlobal, runLen;
@@ -555,20 +583,80 @@ struct PartTables {
 
 __device__ __forceinline__ void load_part_tables(const MeshJob &J, const PartDesc &d, int tid, PartTables &t) {
   const int i = min(tid, d.nr - 1);
-  t.row = J.rows[d.r0 + i];
-  t.lbase = J.rowLbase[d.r0 + i];
-  const int4 r = J.runs[d.rb + min(tid, d.nruns - 1)];
+  t.row = gmem(J.rows)[d.r0 + i];
+  t.lbase = gmem(J.rowLbase)[d.r0 + i];
+  const int4 r = gmem(J.runs)[d.rb + min(tid, d.nruns - 1)];
   t.runLds = r.x;
   t.runGlobal = r.y;
   t.runLen = r.z;
 }
 
-// Job index of global part gp, searching forward from job j (parts of a job are contiguous).
-__device__ __forceinline__ int advance_job(const OwnerArgs &a, int numJobs, int j, int gp) {
-  while (j + 1 < numJobs && a.jobStart[j + 1] <= gp) j++;
-  return j;
+// Parts [p0, p1) of one job, software-pipelined: at the top of the step for part p the workgroup issues
+// part p+1's coordinate/state/rank gathers (its node ids arrived during the previous step), part p+2's node
+// ids, part p+1's row tables and part p+3's descriptor, then computes part p from registers and LDS only;
+// the barriers of a step wait for LDS traffic only, so no wait on HBM falls inside the compute phase.
+template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC>
+__device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, WaveLDS<P, T> &W, const Basis1D &basis,
+                                             const double2 *gref, double *acc, double *racc, int *lrows, int *lbase,
+                                             int4 *lruns) {
+  using O = Ord<P>;
+  constexpr int THREADS = WAVES * 64;
+  const int tid = threadIdx.x, lane = tid % 64;
+  const int el = lane / O::N, idx = lane % O::N;
+  const bool laneOk = lane < O::LANES;
+  const int myv = (tid / 64) * O::EPW + el;  // this lane's element visit within a part
+  const Material mat = job_material(J);
+  const int plast = p1 - 1;
+
+  PartDesc dc = part_desc(J, p0);
+  PartDesc dn = part_desc(J, min(p0 + 1, plast));
+  PartDesc dnn = part_desc(J, min(p0 + 2, plast));
+  PartTables tc;
+  load_part_tables(J, dc, tid, tc);
+  Gather<P> gc, gn;
+  gather_level1<P>(J, dc.e0 + min(myv, dc.ne - 1), idx, gc);
+  gather_level1<P>(J, dn.e0 + min(myv, dn.ne - 1), idx, gn);
+  gather_level2<P>(J, idx, gc);
+
+  for (int p = p0; p < p1; p++) {
+    lds_barrier();  // the previous part's write-out has read the accumulators
+    if (tid < dc.nr) {
+      lrows[tid] = tc.row;
+      lbase[tid] = tc.lbase;
+    }
+    if (JAC) {
+      if (tid < dc.nruns) lruns[tid] = make_int4(tc.runLds, tc.runGlobal, tc.runLen, 0);
+      for (int i = tid; i < 2 * dc.nb; i += THREADS) reinterpret_cast<double2 *>(acc)[i] = make_double2(0, 0);
+    }
+    if (RES) {
+      for (int i = tid; i < dc.nr; i += THREADS) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
+    }
+    // Prefetch (all loads of this step are issued here, before the compute phase).
+    const PartDesc d3 = part_desc(J, min(p + 3, plast));
+    Gather<P> gnn;
+    gather_level2<P>(J, idx, gn);
+    PartTables tn;
+    load_part_tables(J, dn, tid, tn);
+    gather_level1<P>(J, dnn.e0 + min(myv, dnn.ne - 1), idx, gnn);
+    lds_barrier();
+
+    const bool ok = laneOk && myv < dc.ne;
+    owner_batch<P, T, RES, JAC, false>(mat, W, basis, gref, acc, racc, lrows, lbase, dc.nr, gc, ok, el, idx);
+    lds_barrier();
+    owner_write<RES, JAC, WAVES>(J, acc, racc, lruns, dc.nruns, lrows, dc.nr);
+
+    dc = dn;
+    dn = dnn;
+    dnn = d3;
+    tc = tn;
+    gc = gn;
+    gn = gnn;
+  }
 }
 
+// Streaming kernel for parts that take one batch per wavefront (the default partition), over one or
+// many jobs (a batch of resident meshes of one order). A persistent workgroup walks a contiguous range
+// of global parts, job by job; job fields and material sit in scalar registers for the job's parts.
 template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC>
 __global__ __launch_bounds__(WAVES * 64) void owner_stream_kernel(const OwnerArgs args) {
   using O = Ord<P>;
@@ -585,21 +673,19 @@ __global__ __launch_bounds__(WAVES * 64) void owner_stream_kernel(const OwnerArg
   // Contiguous range of global parts per workgroup; workgroup w runs on XCD w % 8, and each XCD's
   // workgroups share one contiguous (spatially adjacent, Morton-ordered) stretch of parts.
   const int xcd = int(blockIdx.x % 8), slot = int(blockIdx.x / 8), perX = int(gridDim.x / 8);
-  const int numJobs = args.counts ? args.counts[0] : args.numJobs;
-  const int64_t np = args.counts ? args.counts[1] : args.totalParts;
+  const int numJobs = uniform(args.counts ? args.counts[0] : args.numJobs);
+  const int64_t np = uniform(args.counts ? args.counts[1] : args.totalParts);
   const int xb = int(np * xcd / 8), xe = int(np * (xcd + 1) / 8);
   const int pb = xb + int(int64_t(xe - xb) * slot / perX);
   const int pe = xb + int(int64_t(xe - xb) * (slot + 1) / perX);
   if (pb >= pe) return;
 
-  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-  const int el = lane / O::N, idx = lane % O::N;
-  const bool laneOk = lane < O::LANES;
-  const int myv = wave * O::EPW + el;  // this lane's element visit within a part
-  WaveLDS<P, T> &W = wl[wave];
+  copy_basis(basis, args.basis);
+  fill_gref<P>(gref, args.basis);
+  WaveLDS<P, T> &W = wl[threadIdx.x / 64];
 
-  // first job: binary search of the job start table
-  int jc = 0;
+  // job of the first part: binary search of the job start table
+  int j = 0;
   {
     int lo = 0, hi = numJobs - 1;
     while (lo < hi) {
@@ -607,57 +693,17 @@ __global__ __launch_bounds__(WAVES * 64) void owner_stream_kernel(const OwnerArg
       if (args.jobStart[mid] <= pb) lo = mid;
       else hi = mid - 1;
     }
-    jc = lo;
+    j = uniform(lo);
   }
-  PartDesc dc = part_desc(args.jobs[jc], pb - args.jobStart[jc]);
-  PartTables tc, tn;
-  Gather<P> gc, gn;
-  load_part_tables(args.jobs[jc], dc, tid, tc);
-  gather_level1<P>(args.jobs[jc], dc.e0 + min(myv, dc.ne - 1), idx, gc);
-  gather_level2<P>(args.jobs[jc], idx, gc);
-  int jn = advance_job(args, numJobs, jc, min(pb + 1, pe - 1));
-  PartDesc dn = part_desc(args.jobs[jn], min(pb + 1, pe - 1) - args.jobStart[jn]);
-  copy_basis(basis, args.basis);
-  fill_gref<P>(gref, args.basis);
-
-  for (int gp = pb; gp < pe; gp++) {
-    __syncthreads();  // the previous part's write-out has read the accumulators
-    if (tid < dc.nr) {
-      lrows[tid] = tc.row;
-      lbase[tid] = tc.lbase;
+  for (int gp = pb; gp < pe; j++) {
+    const int js = uniform(args.jobStart[j]);
+    const int je = uniform(min(args.jobStart[j + 1], pe));
+    const MeshJob J = args.jobs[j];
+    if (je > gp) {
+      stream_parts<P, T, WAVES, MAXB, MAXR, RES, JAC>(J, gp - js, je - js, W, basis, gref, acc, racc, lrows, lbase,
+                                                      lruns);
+      gp = je;
     }
-    if (JAC) {
-      if (tid < dc.nruns) lruns[tid] = make_int4(tc.runLds, tc.runGlobal, tc.runLen, 0);
-      for (int i = tid; i < 2 * dc.nb; i += blockDim.x) reinterpret_cast<double2 *>(acc)[i] = make_double2(0, 0);
-    }
-    if (RES) {
-      for (int i = tid; i < dc.nr; i += blockDim.x) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
-    }
-    __syncthreads();
-
-    // Prefetch: part gp+1 tables and node/element ids, part gp+2 descriptor (clamped to the range).
-    const MeshJob &Jn = args.jobs[jn];
-    load_part_tables(Jn, dn, tid, tn);
-    gather_level1<P>(Jn, dn.e0 + min(myv, dn.ne - 1), idx, gn);
-    const int gnn = min(gp + 2, pe - 1);
-    const int jnn = advance_job(args, numJobs, jn, gnn);
-    const PartDesc dnn = part_desc(args.jobs[jnn], gnn - args.jobStart[jnn]);
-
-    const MeshJob &Jc = args.jobs[jc];
-    const bool ok = laneOk && myv < dc.ne;
-    owner_batch<P, T, RES, JAC, false>(job_material(Jc), W, basis, gref, acc, racc, lrows, lbase, dc.nr, gc, ok, el,
-                                       idx);
-
-    gather_level2<P>(Jn, idx, gn);  // part gp+1 coordinates, states, ranks
-    __syncthreads();
-    owner_write<RES, JAC, WAVES>(Jc, acc, racc, lruns, dc.nruns, lrows, dc.nr);
-
-    jc = jn;
-    dc = dn;
-    jn = jnn;
-    dn = dnn;
-    tc = tn;
-    gc = gn;
   }
 }
 
