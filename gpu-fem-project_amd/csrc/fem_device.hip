@@ -216,13 +216,48 @@ __device__ __forceinline__ void stage_geometry(WaveLDS<P, T> &W, const Basis1D &
   *reinterpret_cast<double2 *>(g + 6) = make_double2(double(ts * syy), double(ts * sxy));
 }
 
+// Geometry of one quadrature point as seen by lane (el, b): reference gradient of node b, J^-1 (rows
+// ia, ib), s_q and the weighted stresses s_q*sigma.
+struct PointGeo {
+  double2 g, ia, ib, sc, sd;
+};
+
+template <int P, typename T>
+__device__ __forceinline__ void load_point(const WaveLDS<P, T> &W, const double2 *gref, int el, int b, int q,
+                                           PointGeo &G) {
+  const double *g = &W.geo[el][8 * q];
+  G.g = gref[q * Ord<P>::N + b];
+  G.ia = *reinterpret_cast<const double2 *>(g + 0);
+  G.ib = *reinterpret_cast<const double2 *>(g + 2);
+  G.sc = *reinterpret_cast<const double2 *>(g + 4);
+  G.sd = *reinterpret_cast<const double2 *>(g + 6);
+}
+
+// Physical gradient n = J^-T g of node b at the point, its weighted copy v = s_q n, and the point's
+// residual contribution (B^T sigma s_q) added to (Rx, Ry).
+template <typename T, typename V2>
+__device__ __forceinline__ void point_terms(const PointGeo &G, T &nx, T &ny, V2 &v, T &Rx, T &Ry) {
+  const T gx = T(G.g.x), gy = T(G.g.y);
+  nx = gx * T(G.ia.x) + gy * T(G.ib.x);
+  ny = gx * T(G.ia.y) + gy * T(G.ib.y);
+  const T txx = T(G.sc.y), tyy = T(G.sd.x), txy = T(G.sd.y);
+  Rx += txx * nx + txy * ny;
+  Ry += txy * nx + tyy * ny;
+  const T s = T(G.sc.x);
+  v.x = s * nx;
+  v.y = s * ny;
+}
+
 // Residual entries of node b and (JAC) stiffness blocks for lane (el, b). K is symmetric, so each
 // unordered node pair {a, b} is formed once: lane b takes rows a = b + d (mod N), d = 0..N/2 (for even
 // N the pair at distance N/2 is taken by the lower lane only). emit(row, col, k) receives each block
 // K(row, col), row-major [xx, xy, yx, yy], in FP64; blocks with a != b are emitted for both (a, b)
 // and (b, a) = K(a, b)^T. Every lane of the wavefront must call this (it synchronises the wavefront);
-// `valid` masks the lane's LDS writes and emits. The geometry of quadrature point q+1 is read while
-// the Gram sums of q are accumulated.
+// `valid` masks the lane's LDS writes and emits.
+// Software pipeline over quadrature points: iteration q publishes point q+1's weighted gradients and
+// issues their reads, then accumulates point q's Gram sums from values read one iteration earlier, so
+// LDS latency overlaps the arithmetic. The two publication buffers alternate; a buffer is rewritten two
+// iterations after it was read, and the LDS executes one wavefront's accesses in order.
 template <int P, typename T, bool JAC, typename Emit>
 __device__ __forceinline__ void stage_stiffness(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el,
                                                 int b, bool valid, T &Rx, T &Ry, Emit &&emit) {
@@ -232,55 +267,72 @@ __device__ __forceinline__ void stage_stiffness(WaveLDS<P, T> &W, const double2 
   const T mu = T(m.mu), lam = T(m.lambda), c1 = lam + 2 * mu;
   Rx = 0;
   Ry = 0;
-  int buf = 0;
   T S[D][4];
 #pragma unroll
   for (int i = 0; i < D; i++) S[i][0] = S[i][1] = S[i][2] = S[i][3] = 0;
-  // rows handled by this lane
   int arow[D];
 #pragma unroll
   for (int d = 0; d < D; d++) arow[d] = b + d < N ? b + d : b + d - N;
   const bool lastPair = (N % 2 == 1) || b < H;  // even N: distance-N/2 pair taken by the lower lane
-  double2 gg = gref[b];
-  double2 ia = *reinterpret_cast<const double2 *>(&W.geo[el][0]);
-  double2 ib = *reinterpret_cast<const double2 *>(&W.geo[el][2]);
-  double2 sc = *reinterpret_cast<const double2 *>(&W.geo[el][4]);
-  double2 sd = *reinterpret_cast<const double2 *>(&W.geo[el][6]);
-#pragma unroll 1
-  for (int q = 0; q < Q; q++) {
-    const T gx = T(gg.x), gy = T(gg.y);
-    const T nx = gx * T(ia.x) + gy * T(ib.x);
-    const T ny = gx * T(ia.y) + gy * T(ib.y);
-    const T txx = T(sc.y), tyy = T(sd.x), txy = T(sd.y);
-    Rx += txx * nx + txy * ny;
-    Ry += txy * nx + tyy * ny;
-    V2 v;
+
+  // Gram sums of one point from the rows' weighted gradients wc and this lane's gradient (nx, ny).
+  auto accumulate = [&](const V2(&wc)[D], T nx, T ny) __attribute__((always_inline)) {
     if (JAC) {
-      const T s = T(sc.x);
-      v.x = s * nx;
-      v.y = s * ny;
-    }
-    // next quadrature point's geometry (clamped: the last iteration re-reads point Q-1)
-    const int qn = q + 1 < Q ? q + 1 : q;
-    const double *g = &W.geo[el][8 * qn];
-    gg = gref[qn * N + b];
-    ia = *reinterpret_cast<const double2 *>(g + 0);
-    ib = *reinterpret_cast<const double2 *>(g + 2);
-    sc = *reinterpret_cast<const double2 *>(g + 4);
-    sd = *reinterpret_cast<const double2 *>(g + 6);
-    if (JAC) {
-      if (valid) W.sg[buf][el][b] = v;
-      wave_sync();
 #pragma unroll
       for (int d = 0; d < D; d++) {
-        const V2 w = W.sg[buf][el][arow[d]];
-        S[d][0] = fma(w.x, nx, S[d][0]);
-        S[d][1] = fma(w.y, ny, S[d][1]);
-        S[d][2] = fma(w.x, ny, S[d][2]);
-        S[d][3] = fma(w.y, nx, S[d][3]);
+        S[d][0] = fma(wc[d].x, nx, S[d][0]);
+        S[d][1] = fma(wc[d].y, ny, S[d][1]);
+        S[d][2] = fma(wc[d].x, ny, S[d][2]);
+        S[d][3] = fma(wc[d].y, nx, S[d][3]);
       }
-      buf ^= 1;
     }
+  };
+  // One pipeline step (q + 1 < Q): publish point q+1 from G (its geometry), issue the reads of its rows
+  // into wn and of its geometry successor into G, then accumulate point q from (wc, nx, ny).
+  auto step = [&](int q, const V2(&wc)[D], V2(&wn)[D], PointGeo &G, T &nx, T &ny) __attribute__((always_inline)) {
+    T nx1, ny1;
+    V2 v1;
+    point_terms<T, V2>(G, nx1, ny1, v1, Rx, Ry);
+    load_point<P, T>(W, gref, el, b, q + 2 < Q ? q + 2 : Q - 1, G);
+    if (JAC) {
+      const int buf = (q + 1) & 1;
+      if (valid) W.sg[buf][el][b] = v1;
+      wave_sync();
+#pragma unroll
+      for (int d = 0; d < D; d++) wn[d] = W.sg[buf][el][arow[d]];
+    }
+    accumulate(wc, nx, ny);
+    nx = nx1;
+    ny = ny1;
+  };
+
+  // point 0
+  PointGeo G;
+  load_point<P, T>(W, gref, el, b, 0, G);
+  T nx, ny;
+  V2 v;
+  point_terms<T, V2>(G, nx, ny, v, Rx, Ry);
+  load_point<P, T>(W, gref, el, b, Q > 1 ? 1 : 0, G);
+  V2 wA[D], wB[D];
+  if (JAC) {
+    if (valid) W.sg[0][el][b] = v;
+    wave_sync();
+#pragma unroll
+    for (int d = 0; d < D; d++) wA[d] = W.sg[0][el][arow[d]];
+  }
+  // Points 0..Q-2 publish their successor; unrolled by two so that the register sets alternate instead
+  // of being copied, with the odd step and the last point peeled at compile time.
+  constexpr int PAIRS = (Q - 1) / 2;
+#pragma unroll 1
+  for (int i = 0; i < PAIRS; i++) {
+    step(2 * i, wA, wB, G, nx, ny);
+    step(2 * i + 1, wB, wA, G, nx, ny);
+  }
+  if constexpr ((Q - 1) % 2 == 1) {
+    step(Q - 2, wA, wB, G, nx, ny);
+    accumulate(wB, nx, ny);
+  } else {
+    accumulate(wA, nx, ny);
   }
   if (JAC && valid) {
 #pragma unroll
@@ -435,7 +487,9 @@ __device__ __forceinline__ void gather_level2(const MeshJob &J, int idx, Gather<
   for (int i = 0; i < W; i++) g.rk[i] = r[i];
 }
 
-template <int P, typename T, bool RES, bool JAC, bool DET>
+// The accumulator holds a part's BSR blocks as four planes, acc[c * MAXB + block] for block component c
+// (row-major xx, xy, yx, yy): the lanes of one LDS atomic then spread over all banks.
+template <int P, typename T, int MAXB, bool RES, bool JAC, bool DET>
 __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &W, const Basis1D &basis,
                                             const double2 *gref, double *acc, double *racc, const int *lrows,
                                             const int *lbase, int nr, const Gather<P> &g, bool valid, int el,
@@ -455,13 +509,13 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
   stage_stiffness<P, T, JAC>(W, gref, mat, el, idx, valid, Rx, Ry, [&](int row, int col, const double(&k)[4]) __attribute__((always_inline)) {
     const int ownR = W.own[el][row];
     if (ownR < 0) return;
-    double *dst = acc + 4 * (lbase[ownR] + rank_of(W, el, row, col));
+    double *dst = acc + (lbase[ownR] + rank_of(W, el, row, col));
     if (DET) {
 #pragma unroll
-      for (int c = 0; c < 4; c++) dst[c] += k[c];
+      for (int c = 0; c < 4; c++) dst[c * MAXB] += k[c];
     } else {
 #pragma unroll
-      for (int c = 0; c < 4; c++) atomicAdd(dst + c, k[c]);
+      for (int c = 0; c < 4; c++) atomicAdd(dst + c * MAXB, k[c]);
     }
   });
   if (RES && valid && own >= 0) {
@@ -476,8 +530,8 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
   wave_sync();
 }
 
-// Writes a part's accumulated BSR runs (one wave per run) and owned residual rows.
-template <bool RES, bool JAC, int WAVES>
+// Writes a part's accumulated BSR runs (one wave per run, one block per lane) and owned residual rows.
+template <bool RES, bool JAC, int WAVES, int MAXB>
 __device__ __forceinline__ void owner_write(const MeshJob &J, const double *acc, const double *racc,
                                             const int4 *lruns, int nruns, const int *lrows, int nr) {
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -485,14 +539,28 @@ __device__ __forceinline__ void owner_write(const MeshJob &J, const double *acc,
     const gptr_mut<double2> values = gmem_mut(reinterpret_cast<double2 *>(J.values));
     for (int r = wave; r < nruns; r += WAVES) {
       const int4 run = lruns[r];
-      const double2 *src = reinterpret_cast<const double2 *>(acc) + 2 * run.x;
+      const double *src = acc + run.x;
       const gptr_mut<double2> dst = values + 2 * int64_t(run.y);
-      for (int i = lane; i < 2 * run.z; i += 64) dst[i] = src[i];
+      for (int i = lane; i < run.z; i += 64) {
+        dst[2 * i] = make_double2(src[i], src[i + MAXB]);
+        dst[2 * i + 1] = make_double2(src[i + 2 * MAXB], src[i + 3 * MAXB]);
+      }
     }
   }
   if (RES && J.residual) {
     const gptr_mut<double2> res = gmem_mut(reinterpret_cast<double2 *>(J.residual));
     for (int i = int(threadIdx.x); i < nr; i += WAVES * 64) res[lrows[i]] = reinterpret_cast<const double2 *>(racc)[i];
+  }
+}
+
+// Zeroes the first nb blocks of each accumulator plane.
+template <int THREADS, int MAXB>
+__device__ __forceinline__ void zero_planes(double *acc, int nb) {
+  const int h = (nb + 1) / 2;  // MAXB is even, so the rounded-up pair stays inside the plane
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    double2 *plane = reinterpret_cast<double2 *>(acc + c * MAXB);
+    for (int i = int(threadIdx.x); i < h; i += THREADS) plane[i] = make_double2(0, 0);
   }
 }
 
@@ -531,7 +599,7 @@ __global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args)
   }
   if (JAC) {
     for (int i = threadIdx.x; i < d.nruns; i += blockDim.x) lruns[i] = J.runs[d.rb + i];
-    for (int i = threadIdx.x; i < 2 * d.nb; i += blockDim.x) reinterpret_cast<double2 *>(acc)[i] = make_double2(0, 0);
+    zero_planes<WAVES * 64, MAXB>(acc, d.nb);
   }
   if (RES) {
     for (int i = threadIdx.x; i < d.nr; i += blockDim.x) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
@@ -550,7 +618,7 @@ __global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args)
         Gather<P> g;
         gather_level1<P>(J, min(d.e0 + b + el, vlast), idx, g);
         gather_level2<P>(J, idx, g);
-        owner_batch<P, T, RES, JAC, DET>(mat, W, basis, gref, acc, racc, lrows, lbase, d.nr, g, ok, el, idx);
+        owner_batch<P, T, MAXB, RES, JAC, DET>(mat, W, basis, gref, acc, racc, lrows, lbase, d.nr, g, ok, el, idx);
       }
       __syncthreads();
     }
@@ -568,11 +636,11 @@ __global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args)
       cur = nxt;
       gather_level2<P>(J, idx, cur);
       gather_level1<P>(J, min(d.e0 + vcur + 2 * STEP, vlast), idx, nxt);
-      owner_batch<P, T, RES, JAC, DET>(mat, W, basis, gref, acc, racc, lrows, lbase, d.nr, g, ok, el, idx);
+      owner_batch<P, T, MAXB, RES, JAC, DET>(mat, W, basis, gref, acc, racc, lrows, lbase, d.nr, g, ok, el, idx);
     }
     __syncthreads();
   }
-  owner_write<RES, JAC, WAVES>(J, acc, racc, lruns, d.nruns, lrows, d.nr);
+  owner_write<RES, JAC, WAVES, MAXB>(J, acc, racc, lruns, d.nruns, lrows, d.nr);
 }
 
 // One thread's entries of a part's row tables: owned row `tid` (node, LDS block base) and run `tid`.
@@ -626,7 +694,7 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
     }
     if (JAC) {
       if (tid < dc.nruns) lruns[tid] = make_int4(tc.runLds, tc.runGlobal, tc.runLen, 0);
-      for (int i = tid; i < 2 * dc.nb; i += THREADS) reinterpret_cast<double2 *>(acc)[i] = make_double2(0, 0);
+      zero_planes<THREADS, MAXB>(acc, dc.nb);
     }
     if (RES) {
       for (int i = tid; i < dc.nr; i += THREADS) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
@@ -641,9 +709,9 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
     lds_barrier();
 
     const bool ok = laneOk && myv < dc.ne;
-    owner_batch<P, T, RES, JAC, false>(mat, W, basis, gref, acc, racc, lrows, lbase, dc.nr, gc, ok, el, idx);
+    owner_batch<P, T, MAXB, RES, JAC, false>(mat, W, basis, gref, acc, racc, lrows, lbase, dc.nr, gc, ok, el, idx);
     lds_barrier();
-    owner_write<RES, JAC, WAVES>(J, acc, racc, lruns, dc.nruns, lrows, dc.nr);
+    owner_write<RES, JAC, WAVES, MAXB>(J, acc, racc, lruns, dc.nruns, lrows, dc.nr);
 
     dc = dn;
     dn = dnn;
@@ -658,7 +726,11 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
 // many jobs (a batch of resident meshes of one order). A persistent workgroup walks a contiguous range
 // of global parts, job by job; job fields and material sit in scalar registers for the job's parts.
 template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC>
-__global__ __launch_bounds__(WAVES * 64) void owner_stream_kernel(const OwnerArgs args) {
+#ifndef FEM_STREAM_MIN_WAVES
+#define FEM_STREAM_MIN_WAVES 2  // waves per SIMD the streaming kernel is held to at orders 1-2
+#endif
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(P <= 2 ? FEM_STREAM_MIN_WAVES : 1, 2)))
+void owner_stream_kernel(const OwnerArgs args) {
   using O = Ord<P>;
   static_assert(MAXR <= WAVES * 64, "one owned row per thread");
   __shared__ WaveLDS<P, T> wl[WAVES];
@@ -1175,6 +1247,7 @@ struct fem_batch {
   hipEvent_t staged = nullptr;  // recorded after the copy: the host mirror is rewritten only once it has run
   bool stagedPending = false;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // per precision
+  hipStream_t captureStream = nullptr;             // graphs are recorded here and replayed on any stream
 
   int *dCounts() const { return reinterpret_cast<int *>(dTables); }
   int *dJobStart() const { return reinterpret_cast<int *>(dTables + jobStartOffset); }
@@ -1195,6 +1268,7 @@ void free_batch(fem_batch *b) {
     if (b->stagedPending) (void)hipEventSynchronize(b->staged);
     (void)hipEventDestroy(b->staged);
   }
+  if (b->captureStream) (void)hipStreamDestroy(b->captureStream);
   if (b->dTables) (void)hipFree(b->dTables);
   if (b->hTables) (void)hipHostFree(b->hTables);
 }
@@ -1219,7 +1293,11 @@ OwnerLaunch batch_group_launch(const fem_batch *b, int gi, int precision, bool f
 }
 
 // Graph of the 12 group launches, each reading its job count from device memory (empty groups exit).
-int capture_batch_graph(fem_batch *b, int precision, hipStream_t s) {
+// Recorded on the batch's own stream (capture is not allowed on the null stream); it is replayed on the
+// caller's stream.
+int capture_batch_graph(fem_batch *b, int precision) {
+  if (!b->captureStream) FEM_HIP_CHECK(hipStreamCreateWithFlags(&b->captureStream, hipStreamNonBlocking));
+  hipStream_t s = b->captureStream;
   hipGraph_t graph = nullptr;
   FEM_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (int gi = 0; gi < fem_batch::kGroups; gi++) {
@@ -1334,7 +1412,7 @@ int fem_batch_assemble(fem_batch *batch, int precision, void *stream, double *el
   FEM_HIP_CHECK(hipEventRecord(batch->staged, s));
   batch->stagedPending = true;
   if (useGraph && !batch->graph[precision]) {
-    int rc = capture_batch_graph(batch, precision, s);
+    int rc = capture_batch_graph(batch, precision);
     if (rc) return rc;
   }
   return timed(s, elapsedSeconds, [&]() -> int {

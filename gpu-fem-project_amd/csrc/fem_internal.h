@@ -30,12 +30,22 @@ struct OwnerConfig {
 };
 // Sized so that two workgroups fit the 160 KB LDS of a CU (per-wave staging of 7-9.5 KB, the FP64
 // block accumulator and the row tables).
+#ifdef FEM_OWNER_WIDE
+// One workgroup of 8 waves per CU at orders 1-2, sharing an accumulator twice as large.
+constexpr OwnerConfig owner_config(int order) {
+  return order == 1   ? OwnerConfig{8, 2720, 384, 128}
+         : order == 2 ? OwnerConfig{8, 2720, 320, 56}
+         : order == 3 ? OwnerConfig{4, 1312, 96, 16}
+                      : OwnerConfig{4, 1312, 64, 8};
+}
+#else
 constexpr OwnerConfig owner_config(int order) {
   return order == 1   ? OwnerConfig{4, 1344, 192, 64}
          : order == 2 ? OwnerConfig{4, 1344, 160, 28}
          : order == 3 ? OwnerConfig{4, 1312, 96, 16}
                       : OwnerConfig{4, 1312, 64, 8};
 }
+#endif
 // Residual-only parts: only the per-row residual lives in LDS.
 constexpr int kResidualOnlyMaxRows = 1024;
 
