@@ -402,9 +402,14 @@ def bench_serve(args, F, torch):
 
     check = None
     if not args.no_check:
-        # Served (mixed) against plain FP64 for one residual+Jacobian request on each mesh.
+        # Served (mixed) against plain FP64 for one residual+Jacobian request on each mesh whose BSR values
+        # fit 640 MB (the comparison copies them to the host).
         er = ev = 0.0
+        checked = 0
         for k, p in enumerate(plain):
+            if p["blocks"] > 20_000_000:
+                continue
+            checked += 1
             rid = srv.submit(p["mid"], p["states"][0], True)
             r_host, v_host = srv.read(rid, p["mesh"].num_nodes, p["blocks"])
             srv.release(rid)
@@ -413,7 +418,7 @@ def bench_serve(args, F, torch):
             v_ref = val_buf[:4 * p["blocks"]]
             er = max(er, rel_err(torch, torch.from_numpy(r_host).to(dev), r_ref))
             ev = max(ev, rel_err(torch, torch.from_numpy(v_host).to(dev), v_ref))
-        check = {"max_rel_err_residual_vs_fp64": er, "max_rel_err_values_vs_fp64": ev}
+        check = {"max_rel_err_residual_vs_fp64": er, "max_rel_err_values_vs_fp64": ev, "meshes_checked": checked}
         log(f"checks: {check}")
     st = srv.stats()
     main_load = loads["0.8"]
