@@ -137,6 +137,7 @@ struct __align__(16) WaveLDS {
   double geo[O::EPW][O::GEO_STRIDE];  // per q: J^-1 (row-major), s_q, s_q*sigma_xx, s_q*sigma_yy, s_q*sigma_xy
   int own[O::EPW][O::N];              // owner path: local row of node a in this part, or -1
   uint32_t rk[O::EPW][O::N][O::RPAD / 4];  // owner path: rk[el][b] byte a = slot of column b within row a
+  unsigned char orow[O::EPW][O::N];        // owner path, orders >= 3: the element's owned nodes, compacted
 };
 
 // Column rank of node `col` within BSR row of node `row` for element `el` (owner path).
@@ -499,6 +500,90 @@ __device__ __forceinline__ void gather_level2(const MeshJob &J, int idx, Gather<
   for (int i = 0; i < W; i++) g.rk[i] = r[i];
 }
 
+// Row-restricted stiffness (orders >= 3, where an element is visited by several parts): lane (el, b) forms
+// K(a, b) only for the rows a this part owns, W.orow[el][0..count), in chunks of CH rows, with the geometry
+// re-read per chunk. Over all parts visiting an element this forms each row of K_e once, instead of
+// the whole symmetric K_e per visit. maxCount is the largest count over the wavefront's elements; every
+// lane runs the same number of chunks (the loop synchronises the wavefront). The residual of node b is
+// accumulated in the first chunk. emit(row, col, k) as for stage_stiffness.
+template <int P, typename T, bool JAC, int CH, typename Emit>
+__device__ __forceinline__ void stage_stiffness_rows(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el,
+                                                     int b, bool valid, int count, int maxCount, T &Rx, T &Ry,
+                                                     Emit &&emit) {
+  using O = Ord<P>;
+  constexpr int N = O::N, Q = O::Q;
+  using V2 = typename Vec2<T>::type;
+  const T mu = T(m.mu), lam = T(m.lambda), c1 = lam + 2 * mu;
+  const int elc = el < O::EPW ? el : O::EPW - 1;  // lanes past the last element read a real row
+  Rx = 0;
+  Ry = 0;
+  const int chunks = JAC ? max((maxCount + CH - 1) / CH, 1) : 1;
+#pragma unroll 1
+  for (int c = 0; c < chunks; c++) {
+    T S[CH][4];
+    int arow[CH];
+#pragma unroll
+    for (int j = 0; j < CH; j++) {
+      S[j][0] = S[j][1] = S[j][2] = S[j][3] = 0;
+      arow[j] = W.orow[elc][min(c * CH + j, max(count - 1, 0))];
+    }
+    int buf = 0;
+    double2 gg = gref[b < N ? b : 0];
+    double2 ia = *reinterpret_cast<const double2 *>(&W.geo[elc][0]);
+    double2 ib = *reinterpret_cast<const double2 *>(&W.geo[elc][2]);
+    double2 sc = *reinterpret_cast<const double2 *>(&W.geo[elc][4]);
+    double2 sd = *reinterpret_cast<const double2 *>(&W.geo[elc][6]);
+#pragma unroll 1
+    for (int q = 0; q < Q; q++) {
+      const T gx = T(gg.x), gy = T(gg.y);
+      const T nx = gx * T(ia.x) + gy * T(ib.x);
+      const T ny = gx * T(ia.y) + gy * T(ib.y);
+      if (c == 0) {
+        const T txx = T(sc.y), tyy = T(sd.x), txy = T(sd.y);
+        Rx += txx * nx + txy * ny;
+        Ry += txy * nx + tyy * ny;
+      }
+      V2 v;
+      const T s = T(sc.x);
+      v.x = s * nx;
+      v.y = s * ny;
+      const int qn = q + 1 < Q ? q + 1 : q;
+      const double *g = &W.geo[elc][8 * qn];
+      gg = gref[qn * N + (b < N ? b : 0)];
+      ia = *reinterpret_cast<const double2 *>(g + 0);
+      ib = *reinterpret_cast<const double2 *>(g + 2);
+      sc = *reinterpret_cast<const double2 *>(g + 4);
+      sd = *reinterpret_cast<const double2 *>(g + 6);
+      if (JAC) {
+        if (valid) W.sg[buf][el][b] = v;
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+          const V2 w = W.sg[buf][elc][arow[j]];
+          S[j][0] = fma(w.x, nx, S[j][0]);
+          S[j][1] = fma(w.y, ny, S[j][1]);
+          S[j][2] = fma(w.x, ny, S[j][2]);
+          S[j][3] = fma(w.y, nx, S[j][3]);
+        }
+        buf ^= 1;
+      }
+    }
+    if (JAC && valid) {
+#pragma unroll
+      for (int j = 0; j < CH; j++) {
+        if (c * CH + j >= count) break;
+        double k[4];
+        k[0] = double(c1 * S[j][0] + mu * S[j][1]);
+        k[1] = double(lam * S[j][2] + mu * S[j][3]);
+        k[2] = double(lam * S[j][3] + mu * S[j][2]);
+        k[3] = double(c1 * S[j][1] + mu * S[j][0]);
+        emit(arow[j], b, k);
+      }
+    }
+    wave_sync();  // the next chunk republishes into the same buffers
+  }
+}
+
 // The accumulator holds a part's BSR blocks as four planes, acc[c * MAXB + block] for block component c
 // (row-major xx, xy, yx, yy): the lanes of one LDS atomic then spread over all banks.
 template <int P, typename T, int MAXB, bool RES, bool JAC, bool DET>
@@ -514,11 +599,23 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
 #pragma unroll
     for (int i = 0; i < Ord<P>::RPAD / 4; i++) W.rk[el][idx][i] = g.rk[i];
   }
+  using O = Ord<P>;
+  // Orders >= 3: compact this element's owned rows (lanes of element e are e*N .. e*N+N-1).
+  int count = 0, maxCount = 0;
+  if constexpr (P >= 3) {
+    const unsigned long long m = __ballot(valid && own >= 0);
+    constexpr unsigned long long elMask = (1ull << O::N) - 1;
+#pragma unroll
+    for (int e = 0; e < O::EPW; e++) maxCount = max(maxCount, __popcll((m >> (e * O::N)) & elMask));
+    const unsigned long long mine = el < O::EPW ? (m >> (el * O::N)) & elMask : 0ull;
+    count = __popcll(mine);
+    if (valid && own >= 0) W.orow[el][__popcll(mine & ((1ull << idx) - 1))] = (unsigned char)idx;
+  }
   wave_sync();
   if (valid) stage_geometry<P, T>(W, basis, mat, el, idx);
   wave_sync();
   T Rx, Ry;
-  stage_stiffness<P, T, JAC>(W, gref, mat, el, idx, valid, Rx, Ry, [&](int row, int col, const double(&k)[4]) __attribute__((always_inline)) {
+  auto emit = [&](int row, int col, const double(&k)[4]) __attribute__((always_inline)) {
     const int ownR = W.own[el][row];
     if (ownR < 0) return;
     double *dst = acc + (lbase[ownR] + rank_of(W, el, row, col));
@@ -529,7 +626,12 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
 #pragma unroll
       for (int c = 0; c < 4; c++) atomicAdd(dst + c * MAXB, k[c]);
     }
-  });
+  };
+  if constexpr (P >= 3) {
+    stage_stiffness_rows<P, T, JAC, 8>(W, gref, mat, el, idx, valid, count, maxCount, Rx, Ry, emit);
+  } else {
+    stage_stiffness<P, T, JAC>(W, gref, mat, el, idx, valid, Rx, Ry, emit);
+  }
   if (RES && valid && own >= 0) {
     if (DET) {
       racc[2 * own] += double(Rx);
