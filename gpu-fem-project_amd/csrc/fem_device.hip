@@ -264,8 +264,8 @@ __device__ __forceinline__ void point_terms(const PointGeo &G, T &nx, T &ny, V2 
 // Residual entries of node b and (JAC) stiffness blocks for lane (el, b). K is symmetric, so each
 // unordered node pair {a, b} is formed once: lane b takes rows a = b + d (mod N), d = 0..N/2 (for even
 // N the pair at distance N/2 is taken by the lower lane only). emit(row, col, k) receives each block
-// K(row, col), row-major [xx, xy, yx, yy], in FP64; blocks with a != b are emitted for both (a, b)
-// and (b, a) = K(a, b)^T. Every lane of the wavefront must call this (it synchronises the wavefront);
+// K(row, col), row-major [xx, xy, yx, yy], in FP64, with a slot number: d for block (a, b) and D + d for
+// (b, a) = K(a, b)^T, emitted when a != b. Every lane of the wavefront must call this (it synchronises the wavefront);
 // `valid` masks the lane's LDS writes and emits.
 // Software pipeline over quadrature points: iteration q publishes point q+1's weighted gradients and
 // issues their reads, then accumulates point q's Gram sums from values read one iteration earlier, so
@@ -356,10 +356,10 @@ __device__ __forceinline__ void stage_stiffness(WaveLDS<P, T> &W, const double2 
       k[1] = double(lam * S[d][2] + mu * S[d][3]);
       k[2] = double(lam * S[d][3] + mu * S[d][2]);
       k[3] = double(c1 * S[d][1] + mu * S[d][0]);
-      emit(arow[d], b, k);
+      emit(arow[d], b, d, k);
       if (d != 0) {
         const double kt[4] = {k[0], k[2], k[1], k[3]};
-        emit(b, arow[d], kt);
+        emit(b, arow[d], D + d, kt);
       }
     }
   }
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   wave_sync();
   const int *mel = args.map + int64_t(eid) * N * N;
   T Rx, Ry;
-  stage_stiffness<P, T, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, [&](int row, int col, const double(&k)[4]) __attribute__((always_inline)) {
+  stage_stiffness<P, T, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, [&](int row, int col, int, const double(&k)[4]) __attribute__((always_inline)) {
     double *blk = args.values + mel[row * N + col];
     if (ATOMIC) {
 #pragma unroll
@@ -501,8 +501,8 @@ __device__ __forceinline__ void gather_level2(const MeshJob &J, int idx, Gather<
 }
 
 // Row-restricted stiffness (orders >= 3, where an element is visited by several parts): lane (el, b) forms
-// K(a, b) only for the rows a this part owns, W.orow[el][0..count), in chunks of CH rows, with the geometry
-// re-read per chunk. Over all parts visiting an element this forms each row of K_e once, instead of
+// K(a, b) only for the rows a this part owns, W.orow[el][0..count), in chunks of up to CH rows (rows past
+// the wavefront's largest count are skipped), with the geometry re-read per chunk. Over all parts visiting an element this forms each row of K_e once, instead of
 // the whole symmetric K_e per visit. maxCount is the largest count over the wavefront's elements; every
 // lane runs the same number of chunks (the loop synchronises the wavefront). The residual of node b is
 // accumulated in the first chunk. emit(row, col, k) as for stage_stiffness.
@@ -520,6 +520,7 @@ __device__ __forceinline__ void stage_stiffness_rows(WaveLDS<P, T> &W, const dou
   const int chunks = JAC ? max((maxCount + CH - 1) / CH, 1) : 1;
 #pragma unroll 1
   for (int c = 0; c < chunks; c++) {
+    const int rows = uniform(min(CH, maxCount - c * CH));  // rows of this chunk; the rest are skipped
     T S[CH][4];
     int arow[CH];
 #pragma unroll
@@ -559,11 +560,13 @@ __device__ __forceinline__ void stage_stiffness_rows(WaveLDS<P, T> &W, const dou
         wave_sync();
 #pragma unroll
         for (int j = 0; j < CH; j++) {
-          const V2 w = W.sg[buf][elc][arow[j]];
-          S[j][0] = fma(w.x, nx, S[j][0]);
-          S[j][1] = fma(w.y, ny, S[j][1]);
-          S[j][2] = fma(w.x, ny, S[j][2]);
-          S[j][3] = fma(w.y, nx, S[j][3]);
+          if (j < rows) {
+            const V2 w = W.sg[buf][elc][arow[j]];
+            S[j][0] = fma(w.x, nx, S[j][0]);
+            S[j][1] = fma(w.y, ny, S[j][1]);
+            S[j][2] = fma(w.x, ny, S[j][2]);
+            S[j][3] = fma(w.y, nx, S[j][3]);
+          }
         }
         buf ^= 1;
       }
@@ -577,7 +580,7 @@ __device__ __forceinline__ void stage_stiffness_rows(WaveLDS<P, T> &W, const dou
         k[1] = double(lam * S[j][2] + mu * S[j][3]);
         k[2] = double(lam * S[j][3] + mu * S[j][2]);
         k[3] = double(c1 * S[j][1] + mu * S[j][0]);
-        emit(arow[j], b, k);
+        emit(arow[j], b, -1, k);
       }
     }
     wave_sync();  // the next chunk republishes into the same buffers
@@ -612,13 +615,38 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
     if (valid && own >= 0) W.orow[el][__popcll(mine & ((1ull << idx) - 1))] = (unsigned char)idx;
   }
   wave_sync();
+  // Orders <= 2: LDS accumulator offsets of this lane's emits, read now so that their latency overlaps the
+  // geometry and Gram stages (slot d: block (a, b); slot D + d: block (b, a); -1: row not owned).
+  // Two 16-bit offsets per register: slot d in the low half, slot D + d in the high half, 0xffff for none.
+  constexpr int NS = P <= 2 ? O::D : 1;
+  uint32_t off[NS];
+  if constexpr (P <= 2) {
+    const int elc = el < O::EPW ? el : O::EPW - 1;
+    const uint32_t hiB = own >= 0 ? uint32_t(lbase[own]) << 16 : 0xffff0000u;
+#pragma unroll
+    for (int d = 0; d < O::D; d++) {
+      const int a = idx + d < O::N ? idx + d : idx + d - O::N;
+      const int oa = W.own[elc][a];
+      const uint32_t lo = oa >= 0 ? uint32_t(lbase[oa] + rank_of(W, elc, a, idx)) : 0xffffu;
+      const uint32_t hi = own >= 0 ? hiB + (uint32_t(rank_of(W, elc, idx, a)) << 16) : 0xffff0000u;
+      off[d] = lo | hi;
+    }
+  }
   if (valid) stage_geometry<P, T>(W, basis, mat, el, idx);
   wave_sync();
   T Rx, Ry;
-  auto emit = [&](int row, int col, const double(&k)[4]) __attribute__((always_inline)) {
-    const int ownR = W.own[el][row];
-    if (ownR < 0) return;
-    double *dst = acc + (lbase[ownR] + rank_of(W, el, row, col));
+  auto emit = [&](int row, int col, int slot, const double(&k)[4]) __attribute__((always_inline)) {
+    int o;
+    if (slot >= 0) {
+      const uint32_t w = off[(slot < NS ? slot : slot - NS) % NS];
+      const uint32_t h = slot < NS ? (w & 0xffffu) : (w >> 16);
+      o = h == 0xffffu ? -1 : int(h);
+    } else {
+      const int ownR = W.own[el][row];
+      o = ownR >= 0 ? lbase[ownR] + rank_of(W, el, row, col) : -1;
+    }
+    if (o < 0) return;
+    double *dst = acc + o;
     if (DET) {
 #pragma unroll
       for (int c = 0; c < 4; c++) dst[c * MAXB] += k[c];
@@ -628,7 +656,7 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
     }
   };
   if constexpr (P >= 3) {
-    stage_stiffness_rows<P, T, JAC, 8>(W, gref, mat, el, idx, valid, count, maxCount, Rx, Ry, emit);
+    stage_stiffness_rows<P, T, JAC, O::D>(W, gref, mat, el, idx, valid, count, maxCount, Rx, Ry, emit);
   } else {
     stage_stiffness<P, T, JAC>(W, gref, mat, el, idx, valid, Rx, Ry, emit);
   }
@@ -782,10 +810,10 @@ __device__ __forceinline__ void stage_stiffness_fat(const WL &W, const RowBasis<
       k[1] = double(lam * S[d][2] + mu * S[d][3]);
       k[2] = double(lam * S[d][3] + mu * S[d][2]);
       k[3] = double(c1 * S[d][1] + mu * S[d][0]);
-      emit(arow[d], b, k);
+      emit(arow[d], b, d, k);
       if (d != 0) {
         const double kt[4] = {k[0], k[2], k[1], k[3]};
-        emit(b, arow[d], kt);
+        emit(b, arow[d], D + d, kt);
       }
     }
   }
@@ -805,13 +833,32 @@ __device__ __forceinline__ void owner_batch_fat(const Material &mat, FatWaveLDS<
     for (int i = 0; i < Ord<P>::RPAD / 4; i++) W.rk[el][idx][i] = g.rk[i];
   }
   wave_sync();
+  // LDS accumulator offsets of this lane's emits (slot d: block (a, b); slot D + d: block (b, a)), read now
+  // so that their latency overlaps the geometry and Gram stages.
+  using O = Ord<P>;
+  // Two 16-bit offsets per register: slot d in the low half, slot D + d in the high half, 0xffff for none.
+  uint32_t off[O::D];
+  {
+    const int elc = el < O::EPW ? el : O::EPW - 1;
+    const uint32_t hiB = own >= 0 ? uint32_t(lbase[own]) << 16 : 0xffff0000u;
+#pragma unroll
+    for (int d = 0; d < O::D; d++) {
+      const int a = idx + d < O::N ? idx + d : idx + d - O::N;
+      const int oa = W.own[elc][a];
+      const uint32_t lo = oa >= 0 ? uint32_t(lbase[oa] + rank_of(W, elc, a, idx)) : 0xffffu;
+      const uint32_t hi = own >= 0 ? hiB + (uint32_t(rank_of(W, elc, idx, a)) << 16) : 0xffff0000u;
+      off[d] = lo | hi;
+    }
+  }
   if (valid) stage_geometry<P, T>(W, basis, mat, el, idx);
   wave_sync();
   T Rx, Ry;
-  stage_stiffness_fat<P, T, JAC>(W, rb, mat, el, idx, valid, Rx, Ry, [&](int row, int col, const double(&k)[4]) __attribute__((always_inline)) {
-    const int ownR = W.own[el][row];
-    if (ownR < 0) return;
-    double *dst = acc + (lbase[ownR] + rank_of(W, el, row, col));
+  stage_stiffness_fat<P, T, JAC>(W, rb, mat, el, idx, valid, Rx, Ry, [&](int, int, int slot, const double(&k)[4]) __attribute__((always_inline)) {
+    const uint32_t w = off[slot < O::D ? slot : slot - O::D];
+    const uint32_t h = slot < O::D ? (w & 0xffffu) : (w >> 16);
+    if (h == 0xffffu) return;
+    const int o = int(h);
+    double *dst = acc + o;
     if (DET) {
 #pragma unroll
       for (int c = 0; c < 4; c++) dst[c * MAXB] += k[c];
@@ -1008,9 +1055,9 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
 // of global parts, job by job; job fields and material sit in scalar registers for the job's parts.
 template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC>
 #ifndef FEM_STREAM_MIN_WAVES
-#define FEM_STREAM_MIN_WAVES 2  // waves per SIMD the streaming kernel is held to at orders 1-2
+#define FEM_STREAM_MIN_WAVES 2  // waves per SIMD the streaming kernel is held to
 #endif
-__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(P <= 2 ? FEM_STREAM_MIN_WAVES : 1, 2)))
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(FEM_STREAM_MIN_WAVES, 2)))
 void owner_stream_kernel(const OwnerArgs args) {
   using O = Ord<P>;
   static_assert(MAXR <= WAVES * 64, "one owned row per thread");
