@@ -500,90 +500,108 @@ __device__ __forceinline__ void gather_level2(const MeshJob &J, int idx, Gather<
   for (int i = 0; i < W; i++) g.rk[i] = r[i];
 }
 
-// Row-restricted stiffness (orders >= 3, where an element is visited by several parts): lane (el, b) forms
-// K(a, b) only for the rows a this part owns, W.orow[el][0..count), in chunks of up to CH rows (rows past
-// the wavefront's largest count are skipped), with the geometry re-read per chunk. Over all parts visiting an element this forms each row of K_e once, instead of
-// the whole symmetric K_e per visit. maxCount is the largest count over the wavefront's elements; every
-// lane runs the same number of chunks (the loop synchronises the wavefront). The residual of node b is
-// accumulated in the first chunk. emit(row, col, k) as for stage_stiffness.
+// One chunk of the row-restricted stiffness: rows W.orow[el][base .. base+CH) (clamped to the element's
+// count; rows past it are formed but not emitted). With `first`, the residual of node b is accumulated.
 template <int P, typename T, bool JAC, int CH, typename Emit>
-__device__ __forceinline__ void stage_stiffness_rows(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el,
-                                                     int b, bool valid, int count, int maxCount, T &Rx, T &Ry,
-                                                     Emit &&emit) {
+__device__ __forceinline__ void rows_chunk(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el, int b,
+                                           bool valid, int count, int base, bool first, T &Rx, T &Ry, Emit &&emit) {
   using O = Ord<P>;
   constexpr int N = O::N, Q = O::Q;
   using V2 = typename Vec2<T>::type;
   const T mu = T(m.mu), lam = T(m.lambda), c1 = lam + 2 * mu;
   const int elc = el < O::EPW ? el : O::EPW - 1;  // lanes past the last element read a real row
-  Rx = 0;
-  Ry = 0;
-  const int chunks = JAC ? max((maxCount + CH - 1) / CH, 1) : 1;
-#pragma unroll 1
-  for (int c = 0; c < chunks; c++) {
-    const int rows = uniform(min(CH, maxCount - c * CH));  // rows of this chunk; the rest are skipped
-    T S[CH][4];
-    int arow[CH];
+  const int bc = b < N ? b : 0;
+  T S[CH][4];
+  int arow[CH];
 #pragma unroll
-    for (int j = 0; j < CH; j++) {
-      S[j][0] = S[j][1] = S[j][2] = S[j][3] = 0;
-      arow[j] = W.orow[elc][min(c * CH + j, max(count - 1, 0))];
-    }
-    int buf = 0;
-    double2 gg = gref[b < N ? b : 0];
-    double2 ia = *reinterpret_cast<const double2 *>(&W.geo[elc][0]);
-    double2 ib = *reinterpret_cast<const double2 *>(&W.geo[elc][2]);
-    double2 sc = *reinterpret_cast<const double2 *>(&W.geo[elc][4]);
-    double2 sd = *reinterpret_cast<const double2 *>(&W.geo[elc][6]);
+  for (int j = 0; j < CH; j++) {
+    S[j][0] = S[j][1] = S[j][2] = S[j][3] = 0;
+    arow[j] = W.orow[elc][min(base + j, max(count - 1, 0))];
+  }
+  int buf = 0;
+  double2 gg = gref[bc];
+  double2 ia = *reinterpret_cast<const double2 *>(&W.geo[elc][0]);
+  double2 ib = *reinterpret_cast<const double2 *>(&W.geo[elc][2]);
+  double2 sc = *reinterpret_cast<const double2 *>(&W.geo[elc][4]);
+  double2 sd = *reinterpret_cast<const double2 *>(&W.geo[elc][6]);
 #pragma unroll 1
-    for (int q = 0; q < Q; q++) {
-      const T gx = T(gg.x), gy = T(gg.y);
-      const T nx = gx * T(ia.x) + gy * T(ib.x);
-      const T ny = gx * T(ia.y) + gy * T(ib.y);
-      if (c == 0) {
-        const T txx = T(sc.y), tyy = T(sd.x), txy = T(sd.y);
-        Rx += txx * nx + txy * ny;
-        Ry += txy * nx + tyy * ny;
-      }
-      V2 v;
-      const T s = T(sc.x);
-      v.x = s * nx;
-      v.y = s * ny;
-      const int qn = q + 1 < Q ? q + 1 : q;
-      const double *g = &W.geo[elc][8 * qn];
-      gg = gref[qn * N + (b < N ? b : 0)];
-      ia = *reinterpret_cast<const double2 *>(g + 0);
-      ib = *reinterpret_cast<const double2 *>(g + 2);
-      sc = *reinterpret_cast<const double2 *>(g + 4);
-      sd = *reinterpret_cast<const double2 *>(g + 6);
-      if (JAC) {
-        if (valid) W.sg[buf][el][b] = v;
-        wave_sync();
-#pragma unroll
-        for (int j = 0; j < CH; j++) {
-          if (j < rows) {
-            const V2 w = W.sg[buf][elc][arow[j]];
-            S[j][0] = fma(w.x, nx, S[j][0]);
-            S[j][1] = fma(w.y, ny, S[j][1]);
-            S[j][2] = fma(w.x, ny, S[j][2]);
-            S[j][3] = fma(w.y, nx, S[j][3]);
-          }
-        }
-        buf ^= 1;
-      }
+  for (int q = 0; q < Q; q++) {
+    const T gx = T(gg.x), gy = T(gg.y);
+    const T nx = gx * T(ia.x) + gy * T(ib.x);
+    const T ny = gx * T(ia.y) + gy * T(ib.y);
+    if (first) {
+      const T txx = T(sc.y), tyy = T(sd.x), txy = T(sd.y);
+      Rx += txx * nx + txy * ny;
+      Ry += txy * nx + tyy * ny;
     }
-    if (JAC && valid) {
+    V2 v;
+    const T s = T(sc.x);
+    v.x = s * nx;
+    v.y = s * ny;
+    const int qn = q + 1 < Q ? q + 1 : q;
+    const double *g = &W.geo[elc][8 * qn];
+    gg = gref[qn * N + bc];
+    ia = *reinterpret_cast<const double2 *>(g + 0);
+    ib = *reinterpret_cast<const double2 *>(g + 2);
+    sc = *reinterpret_cast<const double2 *>(g + 4);
+    sd = *reinterpret_cast<const double2 *>(g + 6);
+    if (JAC) {
+      if (valid) W.sg[buf][el][b] = v;
+      wave_sync();
 #pragma unroll
       for (int j = 0; j < CH; j++) {
-        if (c * CH + j >= count) break;
-        double k[4];
-        k[0] = double(c1 * S[j][0] + mu * S[j][1]);
-        k[1] = double(lam * S[j][2] + mu * S[j][3]);
-        k[2] = double(lam * S[j][3] + mu * S[j][2]);
-        k[3] = double(c1 * S[j][1] + mu * S[j][0]);
-        emit(arow[j], b, -1, k);
+        const V2 w = W.sg[buf][elc][arow[j]];
+        S[j][0] = fma(w.x, nx, S[j][0]);
+        S[j][1] = fma(w.y, ny, S[j][1]);
+        S[j][2] = fma(w.x, ny, S[j][2]);
+        S[j][3] = fma(w.y, nx, S[j][3]);
       }
+      buf ^= 1;
     }
-    wave_sync();  // the next chunk republishes into the same buffers
+  }
+  if (JAC && valid) {
+#pragma unroll
+    for (int j = 0; j < CH; j++) {
+      if (base + j >= count) break;
+      double k[4];
+      k[0] = double(c1 * S[j][0] + mu * S[j][1]);
+      k[1] = double(lam * S[j][2] + mu * S[j][3]);
+      k[2] = double(lam * S[j][3] + mu * S[j][2]);
+      k[3] = double(c1 * S[j][1] + mu * S[j][0]);
+      emit(arow[j], b, -1, k);
+    }
+  }
+  wave_sync();  // the next chunk republishes into the same buffers
+}
+
+// Row-restricted stiffness (orders >= 3, where an element is visited by several parts): lane (el, b) forms
+// K(a, b) only for the rows a this part owns, W.orow[el][0..count). maxCount, the largest count over the
+// wavefront's elements, picks the chunk sizes (N/2+1, 8 or 4 rows, each a branch-free loop), so a
+// wavefront of lightly owned elements does little work. Over all parts visiting an element this forms each
+// row of K_e once, instead of the whole symmetric K_e per visit. The residual of node b is accumulated in
+// the first chunk. emit(row, col, slot, k) as for stage_stiffness (slot -1).
+template <int P, typename T, bool JAC, typename Emit>
+__device__ __forceinline__ void stage_stiffness_rows(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el,
+                                                     int b, bool valid, int count, int maxCount, T &Rx, T &Ry,
+                                                     Emit &&emit) {
+  constexpr int CHMAX = Ord<P>::D;
+  Rx = 0;
+  Ry = 0;
+  const int total = JAC ? uniform(max(maxCount, 1)) : 1;
+#pragma unroll 1
+  for (int base = 0; base < total;) {
+    const int rem = total - base;
+    const bool first = base == 0;
+    if (rem > 8) {
+      rows_chunk<P, T, JAC, CHMAX>(W, gref, m, el, b, valid, count, base, first, Rx, Ry, emit);
+      base += CHMAX;
+    } else if (rem > 4) {
+      rows_chunk<P, T, JAC, 8>(W, gref, m, el, b, valid, count, base, first, Rx, Ry, emit);
+      base += 8;
+    } else {
+      rows_chunk<P, T, JAC, 4>(W, gref, m, el, b, valid, count, base, first, Rx, Ry, emit);
+      base += 4;
+    }
   }
 }
 
@@ -615,38 +633,13 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
     if (valid && own >= 0) W.orow[el][__popcll(mine & ((1ull << idx) - 1))] = (unsigned char)idx;
   }
   wave_sync();
-  // Orders <= 2: LDS accumulator offsets of this lane's emits, read now so that their latency overlaps the
-  // geometry and Gram stages (slot d: block (a, b); slot D + d: block (b, a); -1: row not owned).
-  // Two 16-bit offsets per register: slot d in the low half, slot D + d in the high half, 0xffff for none.
-  constexpr int NS = P <= 2 ? O::D : 1;
-  uint32_t off[NS];
-  if constexpr (P <= 2) {
-    const int elc = el < O::EPW ? el : O::EPW - 1;
-    const uint32_t hiB = own >= 0 ? uint32_t(lbase[own]) << 16 : 0xffff0000u;
-#pragma unroll
-    for (int d = 0; d < O::D; d++) {
-      const int a = idx + d < O::N ? idx + d : idx + d - O::N;
-      const int oa = W.own[elc][a];
-      const uint32_t lo = oa >= 0 ? uint32_t(lbase[oa] + rank_of(W, elc, a, idx)) : 0xffffu;
-      const uint32_t hi = own >= 0 ? hiB + (uint32_t(rank_of(W, elc, idx, a)) << 16) : 0xffff0000u;
-      off[d] = lo | hi;
-    }
-  }
   if (valid) stage_geometry<P, T>(W, basis, mat, el, idx);
   wave_sync();
   T Rx, Ry;
-  auto emit = [&](int row, int col, int slot, const double(&k)[4]) __attribute__((always_inline)) {
-    int o;
-    if (slot >= 0) {
-      const uint32_t w = off[(slot < NS ? slot : slot - NS) % NS];
-      const uint32_t h = slot < NS ? (w & 0xffffu) : (w >> 16);
-      o = h == 0xffffu ? -1 : int(h);
-    } else {
-      const int ownR = W.own[el][row];
-      o = ownR >= 0 ? lbase[ownR] + rank_of(W, el, row, col) : -1;
-    }
-    if (o < 0) return;
-    double *dst = acc + o;
+  auto emit = [&](int row, int col, int, const double(&k)[4]) __attribute__((always_inline)) {
+    const int ownR = W.own[el][row];
+    if (ownR < 0) return;
+    double *dst = acc + (lbase[ownR] + rank_of(W, el, row, col));
     if (DET) {
 #pragma unroll
       for (int c = 0; c < 4; c++) dst[c * MAXB] += k[c];
@@ -656,7 +649,7 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
     }
   };
   if constexpr (P >= 3) {
-    stage_stiffness_rows<P, T, JAC, O::D>(W, gref, mat, el, idx, valid, count, maxCount, Rx, Ry, emit);
+    stage_stiffness_rows<P, T, JAC>(W, gref, mat, el, idx, valid, count, maxCount, Rx, Ry, emit);
   } else {
     stage_stiffness<P, T, JAC>(W, gref, mat, el, idx, valid, Rx, Ry, emit);
   }
@@ -1383,7 +1376,9 @@ void launch_owner(const OwnerLaunch &L, hipStream_t stream) {
   }
   if constexpr (!DET && !cfg.fat && MAXR <= cfg.waves * 64) {
     if (L.stream) {
-      const int grid = std::min(L.streamGrid, ((L.args.totalParts + 7) / 8) * 8);
+      // streamGrid is two workgroups per CU; 8-wave workgroups fill a CU's LDS alone.
+      const int persistent = cfg.waves >= 8 ? std::max(8, L.streamGrid / 16 * 8) : L.streamGrid;
+      const int grid = std::min(persistent, ((L.args.totalParts + 7) / 8) * 8);
       owner_stream_kernel<P, T, cfg.waves, cfg.maxBlocks, MAXR, RES, JAC><<<grid, cfg.waves * 64, 0, stream>>>(
           L.args);
       return;

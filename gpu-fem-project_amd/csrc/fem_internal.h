@@ -31,27 +31,29 @@ struct OwnerConfig {
 };
 #ifdef FEM_OWNER_FAT
 // Orders 1-2: one workgroup of 4 waves per CU (one wave per SIMD, up to 512 registers per lane) with
-// the whole 160 KB of LDS, so parts are about 2.7x larger than below.
+// the whole 160 KB of LDS and the basis gradients in registers.
 constexpr OwnerConfig owner_config(int order) {
   return order == 1   ? OwnerConfig{4, 3456, 448, 512, 1}
          : order == 2 ? OwnerConfig{4, 3584, 320, 224, 1}
          : order == 3 ? OwnerConfig{4, 1312, 96, 16, 0}
                       : OwnerConfig{4, 1312, 64, 8, 0};
 }
-#elif defined(FEM_OWNER_WIDE)
-// One workgroup of 8 waves per CU at orders 1-2, sharing an accumulator twice as large.
+#elif defined(FEM_OWNER_NARROW)
+// Orders 1-2: two workgroups of 4 waves per CU, each with half the LDS.
 constexpr OwnerConfig owner_config(int order) {
-  return order == 1   ? OwnerConfig{8, 2720, 384, 128, 0}
-         : order == 2 ? OwnerConfig{8, 2720, 320, 56, 0}
+  return order == 1   ? OwnerConfig{4, 1344, 192, 64, 0}
+         : order == 2 ? OwnerConfig{4, 1344, 160, 28, 0}
          : order == 3 ? OwnerConfig{4, 1312, 96, 16, 0}
                       : OwnerConfig{4, 1312, 64, 8, 0};
 }
 #else
-// Sized so that two workgroups fit the 160 KB LDS of a CU (per-wave staging of 7-9.5 KB, the FP64
-// block accumulator and the row tables).
+// Orders 1-2: one workgroup of 8 waves per CU (two per SIMD) sharing an accumulator of 2720 FP64 blocks,
+// so parts are twice as large as with two 4-wave workgroups and fewer elements are visited twice
+// (1.53 visits per element at 1000x1000 order 2, against 1.83). Orders 3-4: two workgroups of 4 waves
+// per CU, each with half the LDS.
 constexpr OwnerConfig owner_config(int order) {
-  return order == 1   ? OwnerConfig{4, 1344, 192, 64, 0}
-         : order == 2 ? OwnerConfig{4, 1344, 160, 28, 0}
+  return order == 1   ? OwnerConfig{8, 2720, 384, 128, 0}
+         : order == 2 ? OwnerConfig{8, 2720, 320, 56, 0}
          : order == 3 ? OwnerConfig{4, 1312, 96, 16, 0}
                       : OwnerConfig{4, 1312, 64, 8, 0};
 }
