@@ -583,7 +583,8 @@ def main():
                    "path": args.path, "precision": args.precision, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "owner_kernel" if plan is not None else "element_kernel",
+                     "kernel": ("owner_kernel" if args.path == "plan-det" else "owner_stream_kernel") if plan is not None
+                     else "element_kernel",
                      "algorithmic_bytes": b_alg, "kernel_ms": kernel_s * 1e3},
         "cpu_baseline": cpu,
         "checks": check,

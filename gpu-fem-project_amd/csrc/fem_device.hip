@@ -502,15 +502,18 @@ __device__ __forceinline__ void gather_level2(const MeshJob &J, int idx, Gather<
 
 // One chunk of the row-restricted stiffness: rows W.orow[el][base .. base+CH) (clamped to the element's
 // count; rows past it are formed but not emitted). With `first`, the residual of node b is accumulated.
+// Software-pipelined over quadrature points like stage_stiffness: iteration q publishes point q+1's
+// weighted gradient of node b and issues the reads of the chunk's rows, then accumulates point q from
+// values read one iteration earlier.
 template <int P, typename T, bool JAC, int CH, typename Emit>
 __device__ __forceinline__ void rows_chunk(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el, int b,
                                            bool valid, int count, int base, bool first, T &Rx, T &Ry, Emit &&emit) {
   using O = Ord<P>;
-  constexpr int N = O::N, Q = O::Q;
+  constexpr int Q = O::Q;
   using V2 = typename Vec2<T>::type;
   const T mu = T(m.mu), lam = T(m.lambda), c1 = lam + 2 * mu;
   const int elc = el < O::EPW ? el : O::EPW - 1;  // lanes past the last element read a real row
-  const int bc = b < N ? b : 0;
+  const int bc = b < O::N ? b : 0;
   T S[CH][4];
   int arow[CH];
 #pragma unroll
@@ -518,46 +521,63 @@ __device__ __forceinline__ void rows_chunk(WaveLDS<P, T> &W, const double2 *gref
     S[j][0] = S[j][1] = S[j][2] = S[j][3] = 0;
     arow[j] = W.orow[elc][min(base + j, max(count - 1, 0))];
   }
-  int buf = 0;
-  double2 gg = gref[bc];
-  double2 ia = *reinterpret_cast<const double2 *>(&W.geo[elc][0]);
-  double2 ib = *reinterpret_cast<const double2 *>(&W.geo[elc][2]);
-  double2 sc = *reinterpret_cast<const double2 *>(&W.geo[elc][4]);
-  double2 sd = *reinterpret_cast<const double2 *>(&W.geo[elc][6]);
-#pragma unroll 1
-  for (int q = 0; q < Q; q++) {
-    const T gx = T(gg.x), gy = T(gg.y);
-    const T nx = gx * T(ia.x) + gy * T(ib.x);
-    const T ny = gx * T(ia.y) + gy * T(ib.y);
-    if (first) {
-      const T txx = T(sc.y), tyy = T(sd.x), txy = T(sd.y);
-      Rx += txx * nx + txy * ny;
-      Ry += txy * nx + tyy * ny;
-    }
-    V2 v;
-    const T s = T(sc.x);
-    v.x = s * nx;
-    v.y = s * ny;
-    const int qn = q + 1 < Q ? q + 1 : q;
-    const double *g = &W.geo[elc][8 * qn];
-    gg = gref[qn * N + bc];
-    ia = *reinterpret_cast<const double2 *>(g + 0);
-    ib = *reinterpret_cast<const double2 *>(g + 2);
-    sc = *reinterpret_cast<const double2 *>(g + 4);
-    sd = *reinterpret_cast<const double2 *>(g + 6);
+  // Point q's terms: residual (first chunk only) and the published weighted gradient.
+  T rx = 0, ry = 0;
+  auto accumulate = [&](const V2(&wc)[CH], T nx, T ny) __attribute__((always_inline)) {
     if (JAC) {
-      if (valid) W.sg[buf][el][b] = v;
-      wave_sync();
 #pragma unroll
       for (int j = 0; j < CH; j++) {
-        const V2 w = W.sg[buf][elc][arow[j]];
-        S[j][0] = fma(w.x, nx, S[j][0]);
-        S[j][1] = fma(w.y, ny, S[j][1]);
-        S[j][2] = fma(w.x, ny, S[j][2]);
-        S[j][3] = fma(w.y, nx, S[j][3]);
+        S[j][0] = fma(wc[j].x, nx, S[j][0]);
+        S[j][1] = fma(wc[j].y, ny, S[j][1]);
+        S[j][2] = fma(wc[j].x, ny, S[j][2]);
+        S[j][3] = fma(wc[j].y, nx, S[j][3]);
       }
-      buf ^= 1;
     }
+  };
+  auto step = [&](int q, const V2(&wc)[CH], V2(&wn)[CH], PointGeo &G, T &nx, T &ny) __attribute__((always_inline)) {
+    T nx1, ny1;
+    V2 v1;
+    point_terms<T, V2>(G, nx1, ny1, v1, rx, ry);
+    load_point<P, T>(W, gref, elc, bc, q + 2 < Q ? q + 2 : Q - 1, G);
+    if (JAC) {
+      const int buf = (q + 1) & 1;
+      if (valid) W.sg[buf][el][b] = v1;
+      wave_sync();
+#pragma unroll
+      for (int j = 0; j < CH; j++) wn[j] = W.sg[buf][elc][arow[j]];
+    }
+    accumulate(wc, nx, ny);
+    nx = nx1;
+    ny = ny1;
+  };
+  PointGeo G;
+  load_point<P, T>(W, gref, elc, bc, 0, G);
+  T nx, ny;
+  V2 v;
+  point_terms<T, V2>(G, nx, ny, v, rx, ry);
+  load_point<P, T>(W, gref, elc, bc, Q > 1 ? 1 : 0, G);
+  V2 wA[CH], wB[CH];
+  if (JAC) {
+    if (valid) W.sg[0][el][b] = v;
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < CH; j++) wA[j] = W.sg[0][elc][arow[j]];
+  }
+  constexpr int PAIRS = (Q - 1) / 2;
+#pragma unroll 1
+  for (int i = 0; i < PAIRS; i++) {
+    step(2 * i, wA, wB, G, nx, ny);
+    step(2 * i + 1, wB, wA, G, nx, ny);
+  }
+  if constexpr ((Q - 1) % 2 == 1) {
+    step(Q - 2, wA, wB, G, nx, ny);
+    accumulate(wB, nx, ny);
+  } else {
+    accumulate(wA, nx, ny);
+  }
+  if (first) {
+    Rx += rx;
+    Ry += ry;
   }
   if (JAC && valid) {
 #pragma unroll
@@ -575,33 +595,21 @@ __device__ __forceinline__ void rows_chunk(WaveLDS<P, T> &W, const double2 *gref
 }
 
 // Row-restricted stiffness (orders >= 3, where an element is visited by several parts): lane (el, b) forms
-// K(a, b) only for the rows a this part owns, W.orow[el][0..count). maxCount, the largest count over the
-// wavefront's elements, picks the chunk sizes (N/2+1, 8 or 4 rows, each a branch-free loop), so a
-// wavefront of lightly owned elements does little work. Over all parts visiting an element this forms each
-// row of K_e once, instead of the whole symmetric K_e per visit. The residual of node b is accumulated in
-// the first chunk. emit(row, col, slot, k) as for stage_stiffness (slot -1).
+// K(a, b) only for the rows a this part owns, W.orow[el][0..count), in chunks of 8 rows (4 in FP64; maxCount,
+// the largest count over the wavefront's elements, sets the number of chunks). Over all parts visiting an
+// element this forms each row of K_e once, instead of the whole symmetric K_e per visit. The residual of
+// node b is accumulated in the first chunk. emit(row, col, slot, k) as for stage_stiffness (slot -1).
 template <int P, typename T, bool JAC, typename Emit>
 __device__ __forceinline__ void stage_stiffness_rows(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el,
                                                      int b, bool valid, int count, int maxCount, T &Rx, T &Ry,
                                                      Emit &&emit) {
-  constexpr int CHMAX = Ord<P>::D;
+  constexpr int CH = sizeof(T) == 8 ? 4 : 8;  // FP64 chunks are halved to stay within 256 registers
   Rx = 0;
   Ry = 0;
-  const int total = JAC ? uniform(max(maxCount, 1)) : 1;
+  const int chunks = JAC ? uniform(max((maxCount + CH - 1) / CH, 1)) : 1;
 #pragma unroll 1
-  for (int base = 0; base < total;) {
-    const int rem = total - base;
-    const bool first = base == 0;
-    if (rem > 8) {
-      rows_chunk<P, T, JAC, CHMAX>(W, gref, m, el, b, valid, count, base, first, Rx, Ry, emit);
-      base += CHMAX;
-    } else if (rem > 4) {
-      rows_chunk<P, T, JAC, 8>(W, gref, m, el, b, valid, count, base, first, Rx, Ry, emit);
-      base += 8;
-    } else {
-      rows_chunk<P, T, JAC, 4>(W, gref, m, el, b, valid, count, base, first, Rx, Ry, emit);
-      base += 4;
-    }
+  for (int c = 0; c < chunks; c++) {
+    rows_chunk<P, T, JAC, CH>(W, gref, m, el, b, valid, count, c * CH, c == 0, Rx, Ry, emit);
   }
 }
 

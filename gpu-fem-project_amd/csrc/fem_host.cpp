@@ -340,17 +340,10 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
   if (rc) return rc;
   out.numColours = numColours;
 
-  // Elements touching each part. Deterministic plans group them by colour then id. Otherwise, at orders
-  // >= 3 (where kernels form only a visit's owned rows and a wavefront runs as many rows as its busiest
-  // element), they are ordered by the number of nodes the part owns, most first, then id.
-  const bool deterministic = (flags & FEM_PLAN_DETERMINISTIC) != 0;
-  std::vector<int> nodePart(static_cast<size_t>(numNodes), -1);
-  for (int p = 0; p < P; p++) {
-    for (int k = out.partRowPtr[p]; k < out.partRowPtr[p + 1]; k++) nodePart[size_t(out.rows[k])] = p;
-  }
+  // Elements touching each part, grouped by colour then id (deterministic plans sum colour by colour).
   std::vector<std::vector<int>> partElems(P);
   parallel_for(P, [&](int64_t b, int64_t e) {
-    std::vector<int> tmp, owned;
+    std::vector<int> tmp;
     for (int64_t p = b; p < e; p++) {
       tmp.clear();
       for (int k = out.partRowPtr[p]; k < out.partRowPtr[p + 1]; k++) {
@@ -359,21 +352,7 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
       }
       std::sort(tmp.begin(), tmp.end());
       tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-      if (deterministic) {
-        std::stable_sort(tmp.begin(), tmp.end(), [&](int x, int y) { return colour[x] < colour[y]; });
-      } else if (order >= 3) {
-        owned.assign(size_t(numElements) > 0 ? tmp.size() : 0, 0);
-        for (size_t i = 0; i < tmp.size(); i++) {
-          const int el = tmp[i];
-          for (int a = 0; a < N; a++) owned[i] += nodePart[size_t(conn[size_t(connPtr[el]) + size_t(a)])] == int(p);
-        }
-        std::vector<size_t> ord(tmp.size());
-        for (size_t i = 0; i < ord.size(); i++) ord[i] = i;
-        std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return owned[x] > owned[y]; });
-        std::vector<int> sorted(tmp.size());
-        for (size_t i = 0; i < ord.size(); i++) sorted[i] = tmp[ord[i]];
-        tmp.swap(sorted);
-      }
+      std::stable_sort(tmp.begin(), tmp.end(), [&](int x, int y) { return colour[x] < colour[y]; });
       partElems[p] = tmp;
     }
   });
