@@ -186,7 +186,7 @@ __device__ __forceinline__ void stage_geometry(WL &W, const Basis1D &B, const Ma
     dpy[k] = B.dphi[k][qy];
   }
   double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
-  T U00 = 0, U01 = 0, U10 = 0, U11 = 0;
+  double U00 = 0, U01 = 0, U10 = 0, U11 = 0;
 #pragma unroll
   for (int ay = 0; ay < NP; ay++) {
 #pragma unroll
@@ -200,33 +200,30 @@ __device__ __forceinline__ void stage_geometry(WL &W, const Basis1D &B, const Ma
       J01 = fma(xy.x, gy, J01);
       J10 = fma(xy.y, gx, J10);
       J11 = fma(xy.y, gy, J11);
-      const T tgx = T(gx), tgy = T(gy), tu = T(uv.x), tv = T(uv.y);
-      U00 = fma(tu, tgx, U00);
-      U01 = fma(tu, tgy, U01);
-      U10 = fma(tv, tgx, U10);
-      U11 = fma(tv, tgy, U11);
+      U00 = fma(uv.x, gx, U00);
+      U01 = fma(uv.x, gy, U01);
+      U10 = fma(uv.y, gx, U10);
+      U11 = fma(uv.y, gy, U11);
     }
   }
   const double detJ = J00 * J11 - J01 * J10;
   const double inv = 1.0 / detJ;
   const double i00 = J11 * inv, i01 = -J01 * inv, i10 = -J10 * inv, i11 = J00 * inv;
-  const T t00 = T(i00), t01 = T(i01), t10 = T(i10), t11 = T(i11);
-  const T G00 = U00 * t00 + U01 * t10;
-  const T G01 = U00 * t01 + U01 * t11;
-  const T G10 = U10 * t00 + U11 * t10;
-  const T G11 = U10 * t01 + U11 * t11;
-  const T mu = T(m.mu), lam = T(m.lambda);
-  const T tr = G00 + G11;
-  const T sxx = 2 * mu * G00 + lam * tr;
-  const T syy = 2 * mu * G11 + lam * tr;
-  const T sxy = mu * (G01 + G10);
+  const double G00 = U00 * i00 + U01 * i10;
+  const double G01 = U00 * i01 + U01 * i11;
+  const double G10 = U10 * i00 + U11 * i10;
+  const double G11 = U10 * i01 + U11 * i11;
+  const double mu = m.mu, lam = m.lambda;
+  const double tr = G00 + G11;
+  const double sxx = 2 * mu * G00 + lam * tr;
+  const double syy = 2 * mu * G11 + lam * tr;
+  const double sxy = mu * (G01 + G10);
   const double s = (B.w[qx] * B.w[qy]) * detJ * m.t;
-  const T ts = T(s);
   double *g = &W.geo[el][8 * q];
   *reinterpret_cast<double2 *>(g + 0) = make_double2(i00, i01);
   *reinterpret_cast<double2 *>(g + 2) = make_double2(i10, i11);
-  *reinterpret_cast<double2 *>(g + 4) = make_double2(s, double(ts * sxx));
-  *reinterpret_cast<double2 *>(g + 6) = make_double2(double(ts * syy), double(ts * sxy));
+  *reinterpret_cast<double2 *>(g + 4) = make_double2(s, s * sxx);
+  *reinterpret_cast<double2 *>(g + 6) = make_double2(s * syy, s * sxy);
 }
 
 // Geometry of one quadrature point as seen by lane (el, b): reference gradient of node b, J^-1 (rows
@@ -249,13 +246,15 @@ __device__ __forceinline__ void load_point(const WaveLDS<P, T> &W, const double2
 // Physical gradient n = J^-T g of node b at the point, its weighted copy v = s_q n, and the point's
 // residual contribution (B^T sigma s_q) added to (Rx, Ry).
 template <typename T, typename V2>
-__device__ __forceinline__ void point_terms(const PointGeo &G, T &nx, T &ny, V2 &v, T &Rx, T &Ry) {
-  const T gx = T(G.g.x), gy = T(G.g.y);
-  nx = gx * T(G.ia.x) + gy * T(G.ib.x);
-  ny = gx * T(G.ia.y) + gy * T(G.ib.y);
-  const T txx = T(G.sc.y), tyy = T(G.sd.x), txy = T(G.sd.y);
-  Rx += txx * nx + txy * ny;
-  Ry += txy * nx + tyy * ny;
+// The residual is accumulated in FP64 in both precision modes; in mixed mode the gradient is rounded to
+// FP32 for the Gram sums only.
+__device__ __forceinline__ void point_terms(const PointGeo &G, T &nx, T &ny, V2 &v, double &Rx, double &Ry) {
+  const double nxd = G.g.x * G.ia.x + G.g.y * G.ib.x;
+  const double nyd = G.g.x * G.ia.y + G.g.y * G.ib.y;
+  Rx += G.sc.y * nxd + G.sd.y * nyd;
+  Ry += G.sd.y * nxd + G.sd.x * nyd;
+  nx = T(nxd);
+  ny = T(nyd);
   const T s = T(G.sc.x);
   v.x = s * nx;
   v.y = s * ny;
@@ -273,7 +272,7 @@ __device__ __forceinline__ void point_terms(const PointGeo &G, T &nx, T &ny, V2 
 // iterations after it was read, and the LDS executes one wavefront's accesses in order.
 template <int P, typename T, bool JAC, typename Emit>
 __device__ __forceinline__ void stage_stiffness(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el,
-                                                int b, bool valid, T &Rx, T &Ry, Emit &&emit) {
+                                                int b, bool valid, double &Rx, double &Ry, Emit &&emit) {
   using O = Ord<P>;
   constexpr int N = O::N, Q = O::Q, H = O::H, D = O::D;
   using V2 = typename Vec2<T>::type;
@@ -408,7 +407,7 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   if (valid) stage_geometry<P, T>(W, basis, args.mat, el, idx);
   wave_sync();
   const int *mel = args.map + int64_t(eid) * N * N;
-  T Rx, Ry;
+  double Rx, Ry;
   stage_stiffness<P, T, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, [&](int row, int col, int, const double(&k)[4]) __attribute__((always_inline)) {
     double *blk = args.values + mel[row * N + col];
     if (ATOMIC) {
@@ -507,7 +506,8 @@ __device__ __forceinline__ void gather_level2(const MeshJob &J, int idx, Gather<
 // values read one iteration earlier.
 template <int P, typename T, bool JAC, int CH, typename Emit>
 __device__ __forceinline__ void rows_chunk(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el, int b,
-                                           bool valid, int count, int base, bool first, T &Rx, T &Ry, Emit &&emit) {
+                                           bool valid, int count, int base, bool first, double &Rx, double &Ry,
+                                           Emit &&emit) {
   using O = Ord<P>;
   constexpr int Q = O::Q;
   using V2 = typename Vec2<T>::type;
@@ -521,8 +521,8 @@ __device__ __forceinline__ void rows_chunk(WaveLDS<P, T> &W, const double2 *gref
     S[j][0] = S[j][1] = S[j][2] = S[j][3] = 0;
     arow[j] = W.orow[elc][min(base + j, max(count - 1, 0))];
   }
-  // Point q's terms: residual (first chunk only) and the published weighted gradient.
-  T rx = 0, ry = 0;
+  // Point q's terms: residual (kept for the first chunk only) and the published weighted gradient.
+  double rx = 0, ry = 0;
   auto accumulate = [&](const V2(&wc)[CH], T nx, T ny) __attribute__((always_inline)) {
     if (JAC) {
 #pragma unroll
@@ -601,8 +601,8 @@ __device__ __forceinline__ void rows_chunk(WaveLDS<P, T> &W, const double2 *gref
 // node b is accumulated in the first chunk. emit(row, col, slot, k) as for stage_stiffness (slot -1).
 template <int P, typename T, bool JAC, typename Emit>
 __device__ __forceinline__ void stage_stiffness_rows(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el,
-                                                     int b, bool valid, int count, int maxCount, T &Rx, T &Ry,
-                                                     Emit &&emit) {
+                                                     int b, bool valid, int count, int maxCount, double &Rx,
+                                                     double &Ry, Emit &&emit) {
   constexpr int CH = sizeof(T) == 8 ? 4 : 8;  // FP64 chunks are halved to stay within 256 registers
   Rx = 0;
   Ry = 0;
@@ -643,7 +643,7 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
   wave_sync();
   if (valid) stage_geometry<P, T>(W, basis, mat, el, idx);
   wave_sync();
-  T Rx, Ry;
+  double Rx, Ry;
   auto emit = [&](int row, int col, int, const double(&k)[4]) __attribute__((always_inline)) {
     const int ownR = W.own[el][row];
     if (ownR < 0) return;
@@ -750,7 +750,7 @@ __device__ __forceinline__ void fill_row_basis(const Basis1D &B, int b, RowBasis
 // the register table and the point's J^-1; the geometry of point q+1 is read while point q accumulates.
 template <int P, typename T, bool JAC, typename WL, typename Emit>
 __device__ __forceinline__ void stage_stiffness_fat(const WL &W, const RowBasis<P, T> &rb, const Material &m, int el,
-                                                    int b, bool valid, T &Rx, T &Ry, Emit &&emit) {
+                                                    int b, bool valid, double &Rx, double &Ry, Emit &&emit) {
   using O = Ord<P>;
   constexpr int N = O::N, Q = O::Q, H = O::H, D = O::D;
   const T mu = T(m.mu), lam = T(m.lambda), c1 = lam + 2 * mu;
@@ -787,9 +787,8 @@ __device__ __forceinline__ void stage_stiffness_fat(const WL &W, const RowBasis<
       nx[d] = gx * i00 + gy * i10;
       ny[d] = gx * i01 + gy * i11;
     }
-    const T txx = T(sc[sl].y), tyy = T(sd[sl].x), txy = T(sd[sl].y);
-    Rx += txx * nx[0] + txy * ny[0];
-    Ry += txy * nx[0] + tyy * ny[0];
+    Rx += sc[sl].y * double(nx[0]) + sd[sl].y * double(ny[0]);
+    Ry += sd[sl].y * double(nx[0]) + sd[sl].x * double(ny[0]);
     if (JAC) {
       const T s = T(sc[sl].x);
       const T wx = s * nx[0], wy = s * ny[0];
@@ -853,7 +852,7 @@ __device__ __forceinline__ void owner_batch_fat(const Material &mat, FatWaveLDS<
   }
   if (valid) stage_geometry<P, T>(W, basis, mat, el, idx);
   wave_sync();
-  T Rx, Ry;
+  double Rx, Ry;
   stage_stiffness_fat<P, T, JAC>(W, rb, mat, el, idx, valid, Rx, Ry, [&](int, int, int slot, const double(&k)[4]) __attribute__((always_inline)) {
     const uint32_t w = off[slot < O::D ? slot : slot - O::D];
     const uint32_t h = slot < O::D ? (w & 0xffffu) : (w >> 16);
