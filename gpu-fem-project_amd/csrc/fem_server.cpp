@@ -72,8 +72,13 @@ struct fem_server {
   int device = 0;
   std::vector<ServerMesh> meshes;
   int64_t residentBytes = 0;
-  int64_t outputBytes = 0;  // device bytes held by the output pool (in use + cached)
-  std::multimap<size_t, void *> freeBuffers;
+  // Request outputs live in one device arena, allocated at the first admission with what the budget
+  // leaves after the resident meshes, and sub-allocated first-fit (no hipMalloc/hipFree while serving:
+  // hipFree would synchronise the device and stall the batches in flight).
+  char *arena = nullptr;
+  int64_t arenaBytes = 0;
+  int64_t outputBytes = 0;                 // bytes of the arena held by admitted requests
+  std::map<int64_t, int64_t> freeRanges;   // arena offset -> length, coalesced
   std::unordered_map<int64_t, std::unique_ptr<Request>> requests;
   std::deque<Request *> queue;
   std::vector<Slot> slots;
@@ -93,29 +98,58 @@ int64_t output_bytes(const ServerMesh &m, bool withJacobian) {
   return int64_t(m.numNodes) * 2 * int64_t(sizeof(double)) + (withJacobian ? m.numBlocks * 4 * 8 : 0);
 }
 
-// Pool allocation under the budget; frees cached buffers when that makes room. Caller holds mu.
-void *pool_alloc(fem_server *s, size_t bytes) {
-  auto it = s->freeBuffers.find(bytes);
-  if (it != s->freeBuffers.end()) {
-    void *p = it->second;
-    s->freeBuffers.erase(it);
-    return p;
+constexpr int64_t kArenaAlign = 256;
+
+int64_t align_up(int64_t n) { return (n + kArenaAlign - 1) / kArenaAlign * kArenaAlign; }
+
+// Creates the output arena with the budget left after the resident meshes. Caller holds mu.
+bool ensure_arena(fem_server *s) {
+  if (s->arena) return true;
+  const int64_t bytes = (s->cfg.hbmBudgetBytes - s->residentBytes) / kArenaAlign * kArenaAlign;
+  if (bytes <= 0) return false;
+  if (hipMalloc(reinterpret_cast<void **>(&s->arena), size_t(bytes)) != hipSuccess) {
+    s->arena = nullptr;
+    return false;
   }
-  while (s->residentBytes + s->outputBytes + int64_t(bytes) > s->cfg.hbmBudgetBytes && !s->freeBuffers.empty()) {
-    auto victim = s->freeBuffers.begin();
-    (void)hipFree(victim->second);
-    s->outputBytes -= int64_t(victim->first);
-    s->freeBuffers.erase(victim);
-  }
-  if (s->residentBytes + s->outputBytes + int64_t(bytes) > s->cfg.hbmBudgetBytes) return nullptr;
-  void *p = nullptr;
-  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-  s->outputBytes += int64_t(bytes);
-  return p;
+  s->arenaBytes = bytes;
+  s->freeRanges.clear();
+  s->freeRanges.emplace(0, bytes);
+  return true;
 }
 
+// First-fit allocation from the arena. Caller holds mu.
+void *pool_alloc(fem_server *s, size_t bytes) {
+  if (!ensure_arena(s)) return nullptr;
+  const int64_t need = align_up(int64_t(bytes));
+  for (auto it = s->freeRanges.begin(); it != s->freeRanges.end(); ++it) {
+    if (it->second < need) continue;
+    const int64_t off = it->first, len = it->second;
+    s->freeRanges.erase(it);
+    if (len > need) s->freeRanges.emplace(off + need, len - need);
+    s->outputBytes += need;
+    return s->arena + off;
+  }
+  return nullptr;
+}
+
+// Returns a range to the arena, merging it with free neighbours. Caller holds mu.
 void pool_free(fem_server *s, void *p, size_t bytes) {
-  if (p) s->freeBuffers.emplace(bytes, p);
+  if (!p) return;
+  int64_t off = static_cast<char *>(p) - s->arena, len = align_up(int64_t(bytes));
+  s->outputBytes -= len;
+  auto next = s->freeRanges.lower_bound(off);
+  if (next != s->freeRanges.end() && off + len == next->first) {
+    len += next->second;
+    next = s->freeRanges.erase(next);
+  }
+  if (next != s->freeRanges.begin()) {
+    auto prev = std::prev(next);
+    if (prev->first + prev->second == off) {
+      prev->second += len;
+      return;
+    }
+  }
+  s->freeRanges.emplace(off, len);
 }
 
 // Admits queued requests in FIFO order while their outputs fit the budget. Caller holds mu.
@@ -290,11 +324,12 @@ int fem_server_add_mesh(fem_server *server, int order, int numNodes, int numElem
   m.residentBytes = planBytes + int64_t(numNodes) * 16;
   {
     std::lock_guard<std::mutex> lk(server->mu);
-    if (server->residentBytes + server->outputBytes + m.residentBytes > server->cfg.hbmBudgetBytes) {
+    const int64_t held = server->residentBytes + (server->arena ? server->arenaBytes : 0);
+    if (held + m.residentBytes > server->cfg.hbmBudgetBytes) {
       fem_plan_destroy(m.plan);
-      return set_error(FEM_ERR_BUDGET, "mesh needs %lld resident bytes; budget leaves %lld",
-                       (long long)m.residentBytes,
-                       (long long)(server->cfg.hbmBudgetBytes - server->residentBytes - server->outputBytes));
+      return set_error(FEM_ERR_BUDGET, "mesh needs %lld resident bytes; budget leaves %lld%s",
+                       (long long)m.residentBytes, (long long)(server->cfg.hbmBudgetBytes - held),
+                       server->arena ? " (the output arena was sized at the first admission)" : "");
     }
     server->residentBytes += m.residentBytes;
   }
@@ -329,9 +364,10 @@ int fem_server_submit(fem_server *server, int meshId, const double *states, int 
   if (meshId < 0 || meshId >= int(server->meshes.size())) return set_error(FEM_ERR_INVALID_ARGUMENT, "bad mesh id");
   const ServerMesh &m = server->meshes[size_t(meshId)];
   const int64_t need = output_bytes(m, withJacobian != 0);
-  if (server->residentBytes + need > server->cfg.hbmBudgetBytes) {
+  const int64_t room = server->arena ? server->arenaBytes : server->cfg.hbmBudgetBytes - server->residentBytes;
+  if (align_up(int64_t(m.numNodes) * 16) + (withJacobian ? align_up(m.numBlocks * 32) : 0) > room) {
     return set_error(FEM_ERR_BUDGET, "request outputs need %lld bytes; the budget leaves %lld after resident meshes",
-                     (long long)need, (long long)(server->cfg.hbmBudgetBytes - server->residentBytes));
+                     (long long)need, (long long)room);
   }
   auto r = std::make_unique<Request>();
   r->id = server->nextId++;
@@ -419,7 +455,7 @@ int fem_server_stats(fem_server *server, int64_t *batches, int64_t *completed, i
   std::lock_guard<std::mutex> lk(server->mu);
   if (batches) *batches = server->batches;
   if (completed) *completed = server->completed;
-  if (bytesInUse) *bytesInUse = server->residentBytes + server->outputBytes;
+  if (bytesInUse) *bytesInUse = server->residentBytes + server->outputBytes;  // arena bytes held by requests
   return FEM_OK;
 }
 
@@ -437,12 +473,7 @@ int fem_server_destroy(fem_server *server) {
     if (slot.done) (void)hipEventDestroy(slot.done);
     if (slot.stream) (void)hipStreamDestroy(slot.stream);
   }
-  for (auto &kv : server->requests) {
-    Request *r = kv.second.get();
-    if (r->residual) (void)hipFree(r->residual);
-    if (r->values) (void)hipFree(r->values);
-  }
-  for (auto &kv : server->freeBuffers) (void)hipFree(kv.second);
+  if (server->arena) (void)hipFree(server->arena);
   for (ServerMesh &m : server->meshes) {
     fem_plan_destroy(m.plan);
     (void)hipFree(m.coords);
