@@ -104,10 +104,6 @@ __device__ __forceinline__ gptr_mut<T> gmem_mut(T *p) {
   return (gptr_mut<T>)p;
 }
 
-// s_waitcnt immediate for vmcnt(0) with expcnt and lgkmcnt left at their maxima (gfx9 encoding:
-// vmcnt bits [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]).
-constexpr int kWaitVmcnt0 = 0x0f70;
-
 // Workgroup barrier for LDS traffic only: waits for this wave's LDS operations, then s_barrier. Unlike
 // __syncthreads() it does not wait for outstanding global loads and stores, so prefetches stay in flight
 // across it. The memory clobber keeps the compiler from moving LDS accesses across it.
@@ -700,53 +696,6 @@ __device__ __forceinline__ void owner_write(const MeshJob &J, const double *acc,
   }
 }
 
-// Run of LDS block j: the last run starting at or before j (runs tile the part's blocks in order).
-__device__ __forceinline__ int run_of_block(const int4 *lruns, int nruns, int j) {
-  int lo = 0, hi = nruns - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) / 2;
-    if (lruns[mid].x <= j) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-// Writes a part with a fixed number of stores per thread: thread t writes LDS blocks [t*K, t*K + K)
-// (clamped to the last block, so lanes past the end rewrite it with the same data) and the residual
-// rows t, t + THREADS, ... (clamped likewise). A fixed store count lets the compiler wait for loads
-// issued before the write-out without waiting for its stores, which then drain while the next part is
-// set up and computed. r0 = run_of_block(lruns, nruns, min(t*K, nb - 1)). The launch's outputs decide
-// RES and JAC, so the output pointers are not NULL here.
-template <bool RES, bool JAC, int THREADS, int MAXB, int MAXR>
-__device__ __forceinline__ void owner_write_flat(const MeshJob &J, const double *acc, const double *racc,
-                                                 const int4 *lruns, int nruns, int nb, const int *lrows, int nr,
-                                                 int r0) {
-  const int tid = threadIdx.x;
-  if constexpr (JAC) {
-    constexpr int K = (MAXB + THREADS - 1) / THREADS;
-    const gptr_mut<double2> values = gmem_mut(reinterpret_cast<double2 *>(J.values));
-    int r = r0;
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      const int j = min(tid * K + k, nb - 1);
-      while (r + 1 < nruns && lruns[r + 1].x <= j) r++;
-      const int4 run = lruns[r];
-      const int64_t g = int64_t(run.y) + (j - run.x);
-      values[2 * g] = make_double2(acc[j], acc[j + MAXB]);
-      values[2 * g + 1] = make_double2(acc[j + 2 * MAXB], acc[j + 3 * MAXB]);
-    }
-  }
-  if constexpr (RES) {
-    constexpr int KR = (MAXR + THREADS - 1) / THREADS;
-    const gptr_mut<double2> res = gmem_mut(reinterpret_cast<double2 *>(J.residual));
-#pragma unroll
-    for (int k = 0; k < KR; k++) {
-      const int i = min(tid + k * THREADS, nr - 1);
-      res[lrows[i]] = reinterpret_cast<const double2 *>(racc)[i];
-    }
-  }
-}
-
 // Zeroes the first nb blocks of each accumulator plane.
 template <int THREADS, int MAXB>
 __device__ __forceinline__ void zero_planes(double *acc, int nb) {
@@ -1064,9 +1013,6 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
   gather_level1<P>(J, dc.e0 + min(myv, dc.ne - 1), idx, gc);
   gather_level1<P>(J, dn.e0 + min(myv, dn.ne - 1), idx, gn);
   gather_level2<P>(J, idx, gc);
-  // Complete the prologue's loads here (once per job): with nothing outstanding on entry, the waits
-  // inside the loop only count the previous step's loads and can let the write-out's stores drain.
-  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
 
   for (int p = p0; p < p1; p++) {
     lds_barrier();  // the previous part's write-out has read the accumulators
@@ -1089,13 +1035,11 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
     load_part_tables(J, dn, tid, tn);
     gather_level1<P>(J, dnn.e0 + min(myv, dnn.ne - 1), idx, gnn);
     lds_barrier();
-    constexpr int KW = (MAXB + THREADS - 1) / THREADS;
-    const int r0 = JAC ? run_of_block(lruns, dc.nruns, min(tid * KW, dc.nb - 1)) : 0;
 
     const bool ok = laneOk && myv < dc.ne;
     owner_batch<P, T, MAXB, RES, JAC, false>(mat, W, basis, gref, acc, racc, lrows, lbase, dc.nr, gc, ok, el, idx);
     lds_barrier();
-    owner_write_flat<RES, JAC, THREADS, MAXB, MAXR>(J, acc, racc, lruns, dc.nruns, dc.nb, lrows, dc.nr, r0);
+    owner_write<RES, JAC, WAVES, MAXB>(J, acc, racc, lruns, dc.nruns, lrows, dc.nr);
 
     dc = dn;
     dn = dnn;
