@@ -133,6 +133,7 @@ struct __align__(16) WaveLDS {
   union {
     double nodal[O::EPW][O::NODAL_STRIDE];  // x, y, u, v of each element node
     V2 sg[2][O::EPW][SG_STRIDE];            // s_q * gradN_b(q) of the current quadrature point (double-buffered)
+    float2 ns[O::EPW][4][O::N];             // MFMA stage: gradN_b at four quadrature points
   };
   double geo[O::EPW][O::GEO_STRIDE];  // per q: J^-1 (row-major), s_q, s_q*sigma_xx, s_q*sigma_yy, s_q*sigma_xy
   int own[O::EPW][O::N];              // owner path: local row of node a in this part, or -1
@@ -613,6 +614,125 @@ __device__ __forceinline__ void stage_stiffness_rows(WaveLDS<P, T> &W, const dou
   }
 }
 
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// MFMA form of the row-restricted stiffness (mixed precision, orders >= 3). For each chunk of 8 owned rows
+// of an element, the Gram sums are one product C = A B with
+//   A[(a, alpha)][q] = s_q dN_a/dx_alpha(q)   (16 rows: 8 nodes x 2 components),
+//   B[q][(b, beta)]  = dN_b/dx_beta(q)        (2N columns, in tiles of 16),
+// so C[(a, alpha)][(b, beta)] = S_alpha,beta(a, b), accumulated over the quadrature points four at a time
+// by v_mfma_f32_16x16x4_f32 (lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and holds
+// C[4 (l / 16) + r][l % 16], r = 0..3). Each step, lane (el, b) writes dN_b/dx at four points into LDS; a
+// wavefront then runs the products of each of its elements in turn. Lane pairs (l, l ^ 1) hold the x and
+// y columns of one node b, so after one swap each lane has the four sums of one block (a, b) and emits it.
+// The residual of node b is accumulated in FP64 during the first chunk.
+template <int P, int MAXB, bool DET, typename WL>
+__device__ __forceinline__ void stage_stiffness_mfma(WL &W, const double2 *gref, const Material &m, int el, int b,
+                                                     unsigned long long ownMask, double &Rx, double &Ry, double *acc,
+                                                     const int *lbase) {
+  using O = Ord<P>;
+  constexpr int N = O::N, Q = O::Q, EPW = O::EPW;
+  constexpr int KSTEPS = (Q + 3) / 4, TILES = (2 * N + 15) / 16;
+  constexpr unsigned long long elMask = (1ull << N) - 1;
+  const int lane = int(threadIdx.x % 64);
+  const bool colLane = el < EPW;  // lane (el, b) holds a column node of element el
+  const float mu = float(m.mu), lam = float(m.lambda), c1 = lam + 2 * mu;
+  int cnt[EPW];
+  int maxCount = 0;
+#pragma unroll
+  for (int e = 0; e < EPW; e++) {
+    cnt[e] = __popcll((ownMask >> (e * N)) & elMask);
+    maxCount = max(maxCount, cnt[e]);
+  }
+  Rx = 0;
+  Ry = 0;
+  const int iA = lane % 16, kk = lane / 16;
+  const int chunks = max((maxCount + 7) / 8, 1);
+#pragma unroll 1
+  for (int c = 0; c < chunks; c++) {
+    floatx4 D[EPW][TILES];
+    int arowA[EPW];
+#pragma unroll
+    for (int e = 0; e < EPW; e++) {
+#pragma unroll
+      for (int t = 0; t < TILES; t++) D[e][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+      arowA[e] = W.orow[e][min(c * 8 + iA / 2, max(cnt[e] - 1, 0))];
+    }
+#pragma unroll 1
+    for (int st = 0; st < KSTEPS; st++) {
+      if (colLane) {
+#pragma unroll
+        for (int qq = 0; qq < 4; qq++) {
+          const int q = 4 * st + qq;
+          float2 n = make_float2(0.f, 0.f);
+          if (q < Q) {
+            const double *g = &W.geo[el][8 * q];
+            const double2 gg = gref[q * N + b];
+            const double nxd = gg.x * g[0] + gg.y * g[2];
+            const double nyd = gg.x * g[1] + gg.y * g[3];
+            if (c == 0) {
+              Rx += g[5] * nxd + g[7] * nyd;
+              Ry += g[7] * nxd + g[6] * nyd;
+            }
+            n = make_float2(float(nxd), float(nyd));
+          }
+          W.ns[el][qq][b] = n;
+        }
+      }
+      wave_sync();
+      const int q = 4 * st + kk;
+#pragma unroll
+      for (int e = 0; e < EPW; e++) {
+        float a = 0.f;
+        if (q < Q) {
+          const float2 na = W.ns[e][kk][arowA[e]];
+          a = float(W.geo[e][8 * q + 4]) * ((iA & 1) ? na.y : na.x);
+        }
+#pragma unroll
+        for (int t = 0; t < TILES; t++) {
+          const int col = 16 * t + iA;
+          float bv = 0.f;
+          if (col < 2 * N) {
+            const float2 nb = W.ns[e][kk][col / 2];
+            bv = (col & 1) ? nb.y : nb.x;
+          }
+          D[e][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, D[e][t], 0, 0, 0);
+        }
+      }
+      wave_sync();  // the next step rewrites the slice
+    }
+    const bool even = (lane & 1) == 0;
+    const int jr = 2 * (lane / 16) + (even ? 0 : 1);  // chunk row this lane emits
+#pragma unroll
+    for (int e = 0; e < EPW; e++) {
+#pragma unroll
+      for (int t = 0; t < TILES; t++) {
+        const floatx4 d = D[e][t];
+        float o[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) o[r] = __shfl_xor(d[r], 1);
+        const float sxx = even ? d[0] : o[2], syx = even ? d[1] : o[3];
+        const float sxy = even ? o[0] : d[2], syy = even ? o[1] : d[3];
+        const int bcol = (16 * t + lane % 16) / 2;
+        if (bcol < N && c * 8 + jr < cnt[e]) {
+          const int arow = W.orow[e][c * 8 + jr];
+          const int ownR = W.own[e][arow];
+          double *dst = acc + (lbase[ownR] + rank_of(W, e, arow, bcol));
+          const double k[4] = {double(c1 * sxx + mu * syy), double(lam * sxy + mu * syx),
+                               double(lam * syx + mu * sxy), double(c1 * syy + mu * sxx)};
+          if (DET) {
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++) dst[cc * MAXB] += k[cc];
+          } else {
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++) atomicAdd(dst + cc * MAXB, k[cc]);
+          }
+        }
+      }
+    }
+  }
+}
+
 // The accumulator holds a part's BSR blocks as four planes, acc[c * MAXB + block] for block component c
 // (row-major xx, xy, yx, yy): the lanes of one LDS atomic then spread over all banks.
 template <int P, typename T, int MAXB, bool RES, bool JAC, bool DET>
@@ -631,8 +751,10 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
   using O = Ord<P>;
   // Orders >= 3: compact this element's owned rows (lanes of element e are e*N .. e*N+N-1).
   int count = 0, maxCount = 0;
+  unsigned long long ownMask = 0;
   if constexpr (P >= 3) {
     const unsigned long long m = __ballot(valid && own >= 0);
+    ownMask = m;
     constexpr unsigned long long elMask = (1ull << O::N) - 1;
 #pragma unroll
     for (int e = 0; e < O::EPW; e++) maxCount = max(maxCount, __popcll((m >> (e * O::N)) & elMask));
@@ -656,7 +778,9 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
       for (int c = 0; c < 4; c++) atomicAdd(dst + c * MAXB, k[c]);
     }
   };
-  if constexpr (P >= 3) {
+  if constexpr (P >= 3 && JAC && std::is_same<T, float>::value) {
+    stage_stiffness_mfma<P, MAXB, DET>(W, gref, mat, el, idx, ownMask, Rx, Ry, acc, lbase);
+  } else if constexpr (P >= 3) {
     stage_stiffness_rows<P, T, JAC>(W, gref, mat, el, idx, valid, count, maxCount, Rx, Ry, emit);
   } else {
     stage_stiffness<P, T, JAC>(W, gref, mat, el, idx, valid, Rx, Ry, emit);
