@@ -658,8 +658,9 @@ __device__ __forceinline__ void stage_stiffness_mfma(WL &W, const double2 *gref,
       for (int t = 0; t < TILES; t++) D[e][t] = floatx4{0.f, 0.f, 0.f, 0.f};
       arowA[e] = W.orow[e][min(c * 8 + iA / 2, max(cnt[e] - 1, 0))];
     }
-#pragma unroll 1
-    for (int st = 0; st < KSTEPS; st++) {
+    // Step st: lane (el, b) writes dN_b/dx at points 4st .. 4st+3 (zero past Q); the residual is added in
+    // the first chunk.
+    auto write_slice = [&](int st) __attribute__((always_inline)) {
       if (colLane) {
 #pragma unroll
         for (int qq = 0; qq < 4; qq++) {
@@ -679,28 +680,67 @@ __device__ __forceinline__ void stage_stiffness_mfma(WL &W, const double2 *gref,
           W.ns[el][qq][b] = n;
         }
       }
-      wave_sync();
+    };
+    // MFMA operands of step st for every element of the wavefront.
+    struct Operands {
+      float a[EPW];
+      float bv[EPW][TILES];
+    };
+    auto read_operands = [&](int st, Operands &op) __attribute__((always_inline)) {
       const int q = 4 * st + kk;
 #pragma unroll
       for (int e = 0; e < EPW; e++) {
-        float a = 0.f;
+        op.a[e] = 0.f;
         if (q < Q) {
           const float2 na = W.ns[e][kk][arowA[e]];
-          a = float(W.geo[e][8 * q + 4]) * ((iA & 1) ? na.y : na.x);
+          op.a[e] = float(W.geo[e][8 * q + 4]) * ((iA & 1) ? na.y : na.x);
         }
 #pragma unroll
         for (int t = 0; t < TILES; t++) {
           const int col = 16 * t + iA;
-          float bv = 0.f;
+          op.bv[e][t] = 0.f;
           if (col < 2 * N) {
             const float2 nb = W.ns[e][kk][col / 2];
-            bv = (col & 1) ? nb.y : nb.x;
+            op.bv[e][t] = (col & 1) ? nb.y : nb.x;
           }
-          D[e][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, D[e][t], 0, 0, 0);
         }
       }
-      wave_sync();  // the next step rewrites the slice
+    };
+    auto products = [&](const Operands &op) __attribute__((always_inline)) {
+#pragma unroll
+      for (int e = 0; e < EPW; e++) {
+#pragma unroll
+        for (int t = 0; t < TILES; t++) {
+          D[e][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(op.a[e], op.bv[e][t], D[e][t], 0, 0, 0);
+        }
+      }
+    };
+    // Pipeline: the slice of step st+1 is written and read into registers while step st's products run;
+    // one LDS slice suffices because a wavefront's LDS accesses execute in order.
+    Operands opA, opB;
+    write_slice(0);
+    wave_sync();
+    read_operands(0, opA);
+    auto step = [&](int st, const Operands &cur, Operands &nxt) __attribute__((always_inline)) {
+      wave_sync();  // cur was read before the slice is rewritten (program order)
+      write_slice(st + 1);
+      wave_sync();
+      read_operands(st + 1, nxt);
+      products(cur);
+    };
+    constexpr int PAIRS = (KSTEPS - 1) / 2;
+#pragma unroll 1
+    for (int i = 0; i < PAIRS; i++) {
+      step(2 * i, opA, opB);
+      step(2 * i + 1, opB, opA);
     }
+    if constexpr ((KSTEPS - 1) % 2 == 1) {
+      step(KSTEPS - 2, opA, opB);
+      products(opB);
+    } else {
+      products(opA);
+    }
+    wave_sync();  // the next chunk rewrites the slice
     const bool even = (lane & 1) == 0;
     const int jr = 2 * (lane / 16) + (even ? 0 : 1);  // chunk row this lane emits
 #pragma unroll
