@@ -39,7 +39,7 @@ WORKLOADS = {
 POOL_WORKLOADS = {
     "batch256-mixed": "256 independent meshes (parameter seed 6, mesh i seed 6e6+i): order U{1..4}, elements "
                       "logU[1e3, 2e5], square or annulus 50/50, interior jitter +-20%, u,v ~ U(+-1e-3); mixed "
-                      "precision, one persistent launch per order/output group",
+                      "precision, one persistent launch per order/output group, plans autotuned per mesh",
     "serve-poisson": "serving: Poisson arrivals over a pool of 64 resident meshes (parameter seed 7): order "
                      "U{1..4}, elements logU[1e3, 1e6], square or annulus; each request residual-only or "
                      "residual+Jacobian with 50/50 odds; 200 GB HBM budget; mixed-precision served path vs "
@@ -151,7 +151,8 @@ def bench_batch(args, F, torch):
     for i, m in enumerate(meshes):
         pat = F.bsr_pattern(m)
         u = F.fill_uniform(2 * m.num_nodes, -1e-3, 1e-3, 6 * 10 ** 6 + i)
-        it = dict(mesh=m, blocks=pat.num_blocks, plan=F.Plan(m, pat), coords=d(m.coords), states=d(u),
+        it = dict(mesh=m, blocks=pat.num_blocks, plan=F.Plan(m, pat, flags=F.FEM_PLAN_AUTOTUNE), untuned=F.Plan(m, pat),
+                  coords=d(m.coords), states=d(u),
                   res=torch.zeros(2 * m.num_nodes, dtype=torch.float64, device=dev),
                   vals=torch.zeros(4 * pat.num_blocks, dtype=torch.float64, device=dev),
                   conn=d(m.conn), conn_ptr=d(m.conn_ptr), map=d(F.element_bcsr_map(m, pat)))
@@ -187,7 +188,20 @@ def bench_batch(args, F, torch):
         return (time.perf_counter() - t0) / steps
 
     t_batch = time_steps(lambda: batch.assemble(precision=prec, stream=stream), args.steps)
-    log(f"batch: {t_batch * 1e3:.3f} ms per assembly of all {count} meshes")
+    log(f"batch (autotuned plans): {t_batch * 1e3:.3f} ms per assembly of all {count} meshes")
+    scales = {}
+    for it in items:
+        sc = it["plan"].info()["part_scale"]
+        key = str(it["mesh"].order)
+        scales.setdefault(key, {}).setdefault(str(sc), 0)
+        scales[key][str(sc)] += 1
+    log(f"part scales chosen per order: {scales}")
+    ubatch = F.Batch(max_jobs=count, graph=graph)
+    for it in items:
+        ubatch.add(it["untuned"], it["states"], it["coords"], E_MOD, NU, THICK, res=it["res"], values=it["vals"])
+    t_untuned = time_steps(lambda: ubatch.assemble(precision=prec, stream=stream), args.steps)
+    ubatch.close()
+    log(f"batch (untuned plans): {t_untuned * 1e3:.3f} ms")
 
     def per_mesh_plans():
         for it in items:
@@ -240,7 +254,8 @@ def bench_batch(args, F, torch):
         "roofline": {"bound": "hbm", "achieved": b_alg / t_batch / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": b_alg / t_batch / 1e9 / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": b_alg},
         "baselines": {"one_plan_launch_per_mesh_ms": t_plans * 1e3, "plain_fp64_atomic_per_mesh_ms": t_plain * 1e3,
-                      "speedup_vs_plain": t_plain / t_batch},
+                      "speedup_vs_plain": t_plain / t_batch, "untuned_plans_batch_ms": t_untuned * 1e3},
+        "autotune": {"part_scales_by_order": scales, "candidates": [1.0, 0.5, 0.25]},
         "cpu_baseline": None,
         "checks": check,
     }
@@ -248,6 +263,7 @@ def bench_batch(args, F, torch):
     batch.close()
     for it in items:
         it["plan"].close()
+        it["untuned"].close()
 
 
 def request_stream(F, n, seed, pool_size):

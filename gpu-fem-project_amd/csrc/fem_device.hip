@@ -1766,12 +1766,16 @@ int fem_jacobian_coloured(int order, const int *connPtr, const int *conn, int nu
   });
 }
 
-int fem_plan_create(int order, int numNodes, int numElements, const int *connPtr, const int *conn, const int *rowp,
-                    const int *cols, const double *nodeCoordsHost, int flags, fem_plan **plan) {
-  if (!plan) return set_error(FEM_ERR_NULL_POINTER, "plan is NULL");
+}  // extern "C"
+
+namespace {
+
+int create_plan(int order, int numNodes, int numElements, const int *connPtr, const int *conn, const int *rowp,
+                const int *cols, const double *nodeCoordsHost, int flags, double partScale, fem_plan **plan) {
   *plan = nullptr;
   PlanHost h;
-  int rc = build_plan_host(order, numNodes, numElements, connPtr, conn, rowp, cols, nodeCoordsHost, flags, h);
+  int rc = build_plan_host(order, numNodes, numElements, connPtr, conn, rowp, cols, nodeCoordsHost,
+                           flags & ~FEM_PLAN_AUTOTUNE, h, partScale);
   if (rc) return rc;
   fem_plan *p = new fem_plan();
   p->order = order;
@@ -1804,7 +1808,88 @@ int fem_plan_create(int order, int numNodes, int numElements, const int *connPtr
   bytes += int64_t(sizeof(MeshJob));
   static_assert(sizeof(MeshJob) <= sizeof(p->hostJob), "job record size");
   p->deviceBytes = bytes;
+  p->partScale = partScale;
+  p->flags = flags & ~FEM_PLAN_AUTOTUNE;
   *plan = p;
+  return FEM_OK;
+}
+
+// Device buffers of one timing run, freed on scope exit.
+struct TuneBuffers {
+  double *coords = nullptr, *states = nullptr, *res = nullptr, *vals = nullptr;
+  hipStream_t stream = nullptr;
+  ~TuneBuffers() {
+    if (coords) (void)hipFree(coords);
+    if (states) (void)hipFree(states);
+    if (res) (void)hipFree(res);
+    if (vals) (void)hipFree(vals);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int fem_plan_create(int order, int numNodes, int numElements, const int *connPtr, const int *conn, const int *rowp,
+                    const int *cols, const double *nodeCoordsHost, int flags, fem_plan **plan) {
+  if (!plan) return set_error(FEM_ERR_NULL_POINTER, "plan is NULL");
+  *plan = nullptr;
+  if (!(flags & FEM_PLAN_AUTOTUNE)) {
+    return create_plan(order, numNodes, numElements, connPtr, conn, rowp, cols, nodeCoordsHost, flags, 1.0, plan);
+  }
+  if (flags & (FEM_PLAN_DETERMINISTIC | FEM_PLAN_RESIDUAL_ONLY)) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "FEM_PLAN_AUTOTUNE takes no other plan flag");
+  }
+  if (!nodeCoordsHost) return set_error(FEM_ERR_NULL_POINTER, "FEM_PLAN_AUTOTUNE needs nodeCoordsHost");
+  // Timing inputs: the mesh's coordinates, zero states, outputs of the mesh's size.
+  TuneBuffers tb;
+  const size_t nb = size_t(rowp[numNodes]);
+  FEM_HIP_CHECK(hipStreamCreateWithFlags(&tb.stream, hipStreamNonBlocking));
+  FEM_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&tb.coords), size_t(numNodes) * 16));
+  FEM_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&tb.states), size_t(numNodes) * 16));
+  FEM_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&tb.res), size_t(numNodes) * 16));
+  FEM_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&tb.vals), nb * 32));
+  FEM_HIP_CHECK(hipMemcpy(tb.coords, nodeCoordsHost, size_t(numNodes) * 16, hipMemcpyHostToDevice));
+  FEM_HIP_CHECK(hipMemset(tb.states, 0, size_t(numNodes) * 16));
+  fem_plan *best = nullptr;
+  double bestSeconds = 0;
+  for (const double scale : {1.0, 0.5, 0.25}) {
+    fem_plan *cand = nullptr;
+    int rc = create_plan(order, numNodes, numElements, connPtr, conn, rowp, cols, nodeCoordsHost, flags, scale,
+                         &cand);
+    if (rc) {
+      fem_plan_destroy(best);
+      return rc;
+    }
+    double sec = 0, secMin = 1e30;
+    for (int rep = 0; rep < 3 && !rc; rep++) {  // the first run warms up
+      rc = fem_plan_assemble(cand, tb.states, tb.coords, 1.0, 0.3, 1.0, tb.res, tb.vals, FEM_PREC_MIXED, tb.stream,
+                             &sec);
+      if (rep > 0) secMin = std::min(secMin, sec);
+    }
+    if (rc) {
+      fem_plan_destroy(cand);
+      fem_plan_destroy(best);
+      return rc;
+    }
+    cand->tunedSeconds = secMin;
+    if (!best || secMin < bestSeconds) {
+      fem_plan_destroy(best);
+      best = cand;
+      bestSeconds = secMin;
+    } else {
+      fem_plan_destroy(cand);
+    }
+  }
+  *plan = best;
+  return FEM_OK;
+}
+
+int fem_plan_tuning(const fem_plan *plan, double *partScale, double *seconds) {
+  if (!plan) return set_error(FEM_ERR_NULL_POINTER, "plan is NULL");
+  if (partScale) *partScale = plan->partScale;
+  if (seconds) *seconds = plan->tunedSeconds;
   return FEM_OK;
 }
 

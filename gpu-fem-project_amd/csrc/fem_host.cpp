@@ -172,7 +172,8 @@ void clear_error() { g_lastError.clear(); }
 
 // ---- owner-computes partitioner ------------------------------------------------------------------
 int build_plan_host(int order, int numNodes, int numElements, const int *connPtr, const int *conn, const int *rowp,
-                    const int *cols, const double *nodeCoords, int flags, PlanHost &out) {
+                    const int *cols, const double *nodeCoords, int flags, PlanHost &out,
+                    double partScale) {
   if (!valid_order(order)) return set_error(FEM_ERR_INVALID_ORDER, "element order %d outside 1..4", order);
   const int N = nodes_per_element(order);
   if (!rowp || !cols) return set_error(FEM_ERR_NULL_POINTER, "rowp/cols is NULL");
@@ -255,8 +256,10 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
   // Greedy cut along that order. A part closes when its rows would exceed the LDS block budget, the
   // row table, or the number of element visits one round of the workgroup's wavefronts covers.
   // FEM_AMD_PART_VISITS / FEM_AMD_PART_BLOCKS lower these bounds (tuning; never above the kernel's).
-  int maxVisits = residualOnly ? (1 << 30) : cfg.maxVisits;
-  int blockBudget = maxBlocks;
+  const double scale = std::min(1.0, std::max(partScale, 0.01));
+  int maxVisits = residualOnly ? (1 << 30) : std::max(1, int(cfg.maxVisits * scale));
+  int blockBudget = residualOnly ? maxBlocks : std::max(out.maxRowLength, int(maxBlocks * scale));
+  const int rowBudget = residualOnly ? maxRows : std::max(1, int(maxRows * scale));
   if (const char *ev = std::getenv("FEM_AMD_PART_VISITS")) {
     const int v = std::atoi(ev);
     if (v > 0) maxVisits = v;
@@ -282,7 +285,7 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
       int newVisits = 0;
       for (int j = nptr[n]; j < nptr[n + 1]; j++) newVisits += stamp[nadj[size_t(j)]] != part;
       if (rowsInPart > 0 &&
-          (blocks + len > blockBudget || rowsInPart + 1 > maxRows || visits + newVisits > maxVisits)) {
+          (blocks + len > blockBudget || rowsInPart + 1 > rowBudget || visits + newVisits > maxVisits)) {
         out.partRowPtr.push_back(k);
         part++;
         blocks = 0;

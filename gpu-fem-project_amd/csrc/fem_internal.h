@@ -38,6 +38,14 @@ constexpr OwnerConfig owner_config(int order) {
          : order == 3 ? OwnerConfig{4, 1312, 96, 16, 0}
                       : OwnerConfig{4, 1312, 64, 8, 0};
 }
+#elif defined(FEM_OWNER_WIDE34)
+// All orders: one workgroup of 8 waves per CU sharing an accumulator of 2720 FP64 blocks.
+constexpr OwnerConfig owner_config(int order) {
+  return order == 1   ? OwnerConfig{8, 2720, 384, 128, 0}
+         : order == 2 ? OwnerConfig{8, 2720, 320, 56, 0}
+         : order == 3 ? OwnerConfig{8, 2560, 192, 32, 0}
+                      : OwnerConfig{8, 2560, 128, 16, 0};
+}
 #elif defined(FEM_OWNER_NARROW)
 // Orders 1-2: two workgroups of 4 waves per CU, each with half the LDS.
 constexpr OwnerConfig owner_config(int order) {
@@ -86,8 +94,11 @@ struct PlanHost {
   int maxPartVisits = 0;
 };
 
+// partScale (0, 1] scales the part budgets (blocks, rows, visits) below the kernel's maxima: smaller
+// parts give small meshes more workgroups at the price of more elements recomputed on part boundaries.
 int build_plan_host(int order, int numNodes, int numElements, const int *connPtr, const int *conn,
-                    const int *rowp, const int *cols, const double *nodeCoords, int flags, PlanHost &out);
+                    const int *rowp, const int *cols, const double *nodeCoords, int flags, PlanHost &out,
+                    double partScale = 1.0);
 
 }  // namespace fem
 
@@ -113,6 +124,8 @@ struct fem_plan {
   int *dPartDesc = nullptr;
   uint8_t *dRank = nullptr;
   int *dJobStart = nullptr;  // {0, numParts}
+  double partScale = 1.0;    // part budgets relative to the kernel's (FEM_PLAN_AUTOTUNE picks it)
+  double tunedSeconds = 0;   // measured assembly time of the chosen build (autotuned plans)
   void *dJob = nullptr;      // device job record of the last assembly
   alignas(8) unsigned char hostJob[256] = {};  // host copy of that record
   bool jobUploaded = false;

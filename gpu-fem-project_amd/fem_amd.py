@@ -25,6 +25,7 @@ FEM_PREC_FP64 = 0
 FEM_PREC_MIXED = 1
 FEM_PLAN_DETERMINISTIC = 1
 FEM_PLAN_RESIDUAL_ONLY = 2
+FEM_PLAN_AUTOTUNE = 4
 FEM_BATCH_GRAPH = 1
 
 STATUS_NAMES = {
@@ -70,6 +71,7 @@ SIGNATURES = {
     ),
     "fem_plan_create": ([_I, _I, _I, _P, _P, _P, _P, _P, _I, ctypes.POINTER(_P)], _I),
     "fem_plan_info": ([_P, _PI64, _PI64, _PI64, _PI64], _I),
+    "fem_plan_tuning": ([_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], _I),
     "fem_plan_assemble": ([_P, _P, _P, _D, _D, _D, _P, _P, _I, _P, ctypes.POINTER(_D)], _I),
     "fem_plan_destroy": ([_P], _I),
     "fem_bsr_spmv": ([_I, _P, _P, _P, _P, _P, _P], _I),
@@ -342,7 +344,11 @@ class Plan:
     def info(self):
         vals = [ctypes.c_int64() for _ in range(4)]
         _check(self._lib.fem_plan_info(self.handle, *[ctypes.byref(v) for v in vals]))
-        return dict(zip(["parts", "blocks", "element_visits", "device_bytes"], [v.value for v in vals]))
+        out = dict(zip(["parts", "blocks", "element_visits", "device_bytes"], [v.value for v in vals]))
+        scale, sec = ctypes.c_double(), ctypes.c_double()
+        _check(self._lib.fem_plan_tuning(self.handle, ctypes.byref(scale), ctypes.byref(sec)))
+        out["part_scale"] = scale.value
+        return out
 
     def assemble(self, states, coords, E, nu, t, res=None, values=None, precision=FEM_PREC_FP64, stream=None,
                  timed=False):

@@ -141,6 +141,15 @@ int fem_plan_create(int order, int numNodes, int numElements, const int *connPtr
 /* flags for fem_plan_create */
 #define FEM_PLAN_DETERMINISTIC 1 /* colour-phased LDS sums: bitwise reproducible results */
 #define FEM_PLAN_RESIDUAL_ONLY 2 /* size parts for residual-only assembly */
+#define FEM_PLAN_AUTOTUNE 4      /* build the plan at part sizes 1, 1/2 and 1/4 of the kernel's budget, time one
+                                    mixed-precision residual + Jacobian assembly of each on the device (zero
+                                    states, temporary outputs of the mesh's size) and keep the fastest; smaller
+                                    parts give small meshes more workgroups. Not with the flags above. */
+
+/* Part-size scale a plan was built with (1, 0.5 or 0.25; 1 without FEM_PLAN_AUTOTUNE) and, for an autotuned
+ * plan, the measured seconds of the chosen build (0 otherwise). No counterpart in the reference: the
+ * reference fixes elements per block at compile time (FEKernels.h:1015-1116). */
+int fem_plan_tuning(const fem_plan *plan, double *partScale, double *seconds);
 
 /* What a plan holds: number of parts, BSR blocks, element visits (elements counted once per part
  * they touch), resident device bytes. */

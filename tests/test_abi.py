@@ -77,6 +77,21 @@ def test_plan_create_rejects_bad_meshes_on_host(fem):
                                ctypes.byref(handle)) == -5
 
 
+def test_plan_autotune_rejects_other_flags_and_missing_coords(fem):
+    lib = fem.load()
+    m = fem.square_mesh(2, 2)
+    pat = fem.bsr_pattern(m)
+    handle = ctypes.c_void_p()
+    cp = m.conn_ptr
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    for extra in (fem.FEM_PLAN_DETERMINISTIC, fem.FEM_PLAN_RESIDUAL_ONLY):
+        assert lib.fem_plan_create(2, m.num_nodes, m.num_elements, p(cp), p(m.conn), p(pat.rowp), p(pat.cols),
+                                   p(m.coords), fem.FEM_PLAN_AUTOTUNE | extra, ctypes.byref(handle)) == -3
+    assert lib.fem_plan_create(2, m.num_nodes, m.num_elements, p(cp), p(m.conn), p(pat.rowp), p(pat.cols), None,
+                               fem.FEM_PLAN_AUTOTUNE, ctypes.byref(handle)) == -2
+    assert not handle.value
+
+
 def test_cpp_dropin_header_compiles_against_library():
     # The C++ drop-in demo (include/FEKernels.h + libfem_amd.so) is built by build().
     assert os.path.exists(os.path.join(ROOT, "gpu-fem-project_amd", "fem_dropin_demo"))

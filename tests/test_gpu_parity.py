@@ -215,3 +215,25 @@ def test_cpp_dropin_demo_runs():
         out = subprocess.run([exe, str(order), "12"], capture_output=True, text=True, timeout=120)
         print(out.stdout)
         assert out.returncode == 0, out.stdout + out.stderr
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_autotuned_plan_matches_oracle(fem, orc, torch_gpu, order):
+    """FEM_PLAN_AUTOTUNE keeps the fastest of three part sizes; whichever it keeps, results are the oracle's."""
+    torch = torch_gpu
+    n = {1: 12, 2: 8, 3: 5, 4: 4}[order]
+    m = fem.square_mesh(n, order, jitter=0.2, seed=70 + order)
+    u = fem.fill_uniform(2 * m.num_nodes, -1e-3, 1e-3, 71 + order)
+    pat = fem.bsr_pattern(m)
+    ref_res, ref_vals = orc.jacobian(m, u, E, NU, T, fem.element_bcsr_map(m, pat), pat.num_blocks)
+    plan = fem.Plan(m, pat, flags=fem.FEM_PLAN_AUTOTUNE)
+    info = plan.info()
+    assert info["part_scale"] in (1.0, 0.5, 0.25)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")  # noqa: E731
+    res = torch.full((2 * m.num_nodes,), float("nan"), dtype=torch.float64, device="cuda:0")
+    vals = torch.full((4 * pat.num_blocks,), float("nan"), dtype=torch.float64, device="cuda:0")
+    plan.assemble(d(u), d(m.coords), E, NU, T, res=res, values=vals)
+    torch.cuda.synchronize()
+    assert textbook.normwise_rel(res.cpu().numpy(), ref_res) < 1e-12
+    assert textbook.normwise_rel(vals.cpu().numpy(), ref_vals) < 1e-12
+    plan.close()
