@@ -775,20 +775,38 @@ __device__ __forceinline__ void stage_stiffness_mfma(WL &W, const double2 *gref,
 
 // The accumulator holds a part's BSR blocks as four planes, acc[c * MAXB + block] for block component c
 // (row-major xx, xy, yx, yy): the lanes of one LDS atomic then spread over all banks.
-template <int P, typename T, int MAXB, bool RES, bool JAC, bool DET>
-__device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &W, const Basis1D &basis,
-                                            const double2 *gref, double *acc, double *racc, const int *lrows,
-                                            const int *lbase, int nr, const Gather<P> &g, bool valid, int el,
-                                            int idx) {
-  int own = -1;
+//
+// An element batch runs in three phases: batch_nodes stages the gathered nodes and column ranks,
+// batch_geometry forms each quadrature point's geometry (both use only the wavefront's own LDS), and
+// batch_stiffness marks the part's rows, forms the residual and stiffness and adds them to the part's
+// accumulators. The streaming kernel runs the previous part's write-out between the first two phases.
+template <int P, typename T>
+__device__ __forceinline__ void batch_nodes(WaveLDS<P, T> &W, const Gather<P> &g, bool valid, int el, int idx) {
   if (valid) {
-    own = (g.lrow < nr && lrows[g.lrow] == g.node) ? g.lrow : -1;
     store_nodal(W, el, idx, g.xy, g.uv);
-    W.own[el][idx] = own;
 #pragma unroll
     for (int i = 0; i < Ord<P>::RPAD / 4; i++) W.rk[el][idx][i] = g.rk[i];
   }
+  wave_sync();
+}
+
+template <int P, typename T>
+__device__ __forceinline__ void batch_geometry(WaveLDS<P, T> &W, const Basis1D &basis, const Material &mat, bool valid,
+                                               int el, int idx) {
+  if (valid) stage_geometry<P, T>(W, basis, mat, el, idx);
+  wave_sync();
+}
+
+template <int P, typename T, int MAXB, bool RES, bool JAC, bool DET>
+__device__ __forceinline__ void batch_stiffness(const Material &mat, WaveLDS<P, T> &W, const double2 *gref,
+                                                double *acc, double *racc, const int *lrows, const int *lbase, int nr,
+                                                const Gather<P> &g, bool valid, int el, int idx) {
   using O = Ord<P>;
+  int own = -1;
+  if (valid) {
+    own = (g.lrow < nr && lrows[g.lrow] == g.node) ? g.lrow : -1;
+    W.own[el][idx] = own;
+  }
   // Orders >= 3: compact this element's owned rows (lanes of element e are e*N .. e*N+N-1).
   int count = 0, maxCount = 0;
   unsigned long long ownMask = 0;
@@ -802,8 +820,6 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
     count = __popcll(mine);
     if (valid && own >= 0) W.orow[el][__popcll(mine & ((1ull << idx) - 1))] = (unsigned char)idx;
   }
-  wave_sync();
-  if (valid) stage_geometry<P, T>(W, basis, mat, el, idx);
   wave_sync();
   double Rx, Ry;
   auto emit = [&](int row, int col, int, const double(&k)[4]) __attribute__((always_inline)) {
@@ -835,6 +851,16 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
     }
   }
   wave_sync();
+}
+
+template <int P, typename T, int MAXB, bool RES, bool JAC, bool DET>
+__device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &W, const Basis1D &basis,
+                                            const double2 *gref, double *acc, double *racc, const int *lrows,
+                                            const int *lbase, int nr, const Gather<P> &g, bool valid, int el,
+                                            int idx) {
+  batch_nodes<P, T>(W, g, valid, el, idx);
+  batch_geometry<P, T>(W, basis, mat, valid, el, idx);
+  batch_stiffness<P, T, MAXB, RES, JAC, DET>(mat, W, gref, acc, racc, lrows, lbase, nr, g, valid, el, idx);
 }
 
 // Writes a part's accumulated BSR runs (one wave per run, one block per lane) and owned residual rows.
@@ -1151,10 +1177,11 @@ __device__ __forceinline__ void load_part_tables(const MeshJob &J, const PartDes
   t.runLen = r.z;
 }
 
-// Parts [p0, p1) of one job, software-pipelined: at the top of the step for part p the workgroup issues
-// part p+1's coordinate/state/rank gathers (its node ids arrived during the previous step), part p+2's node
-// ids, part p+1's row tables and part p+3's descriptor, then computes part p from registers and LDS only;
-// the barriers of a step wait for LDS traffic only, so no wait on HBM falls inside the compute phase.
+// Parts [p0, p1) of one job, software-pipelined. A step stages part p's nodes, writes out part p-1 and forms
+// part p's geometry, then (after a barrier) sets up part p's tables and accumulators and issues part p+1's
+// coordinate/state/rank gathers (its node ids arrived during the previous step), part p+2's node ids, part
+// p+1's row tables and part p+3's descriptor, and computes part p's stiffness from registers and LDS only.
+// The barriers wait for LDS traffic only, so no wait on HBM falls inside the compute phases.
 template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC>
 __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, WaveLDS<P, T> &W, const Basis1D &basis,
                                              const double2 *gref, double *acc, double *racc, int *lrows, int *lbase,
@@ -1178,8 +1205,16 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
   gather_level1<P>(J, dn.e0 + min(myv, dn.ne - 1), idx, gn);
   gather_level2<P>(J, idx, gc);
 
+  PartDesc dprev = dc;
   for (int p = p0; p < p1; p++) {
-    lds_barrier();  // the previous part's write-out has read the accumulators
+    const bool ok = laneOk && myv < dc.ne;
+    // Stage this part's nodes (gathered during the previous step), then write out the previous part:
+    // its stores drain while this part's geometry forms (both phases use only the wavefront's own LDS
+    // besides the previous part's accumulators and tables, which nothing writes until the barrier below).
+    batch_nodes<P, T>(W, gc, ok, el, idx);
+    if (p > p0) owner_write<RES, JAC, WAVES, MAXB>(J, acc, racc, lruns, dprev.nruns, lrows, dprev.nr);
+    batch_geometry<P, T>(W, basis, mat, ok, el, idx);
+    lds_barrier();  // the previous part's write-out has read the accumulators and tables
     if (tid < dc.nr) {
       lrows[tid] = tc.row;
       lbase[tid] = tc.lbase;
@@ -1191,7 +1226,7 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
     if (RES) {
       for (int i = tid; i < dc.nr; i += THREADS) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
     }
-    // Prefetch (all loads of this step are issued here, before the compute phase).
+    // Prefetch (all loads of this step are issued here, before the stiffness phase).
     const PartDesc d3 = part_desc(J, min(p + 3, plast));
     Gather<P> gnn;
     gather_level2<P>(J, idx, gn);
@@ -1200,11 +1235,10 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
     gather_level1<P>(J, dnn.e0 + min(myv, dnn.ne - 1), idx, gnn);
     lds_barrier();
 
-    const bool ok = laneOk && myv < dc.ne;
-    owner_batch<P, T, MAXB, RES, JAC, false>(mat, W, basis, gref, acc, racc, lrows, lbase, dc.nr, gc, ok, el, idx);
-    lds_barrier();
-    owner_write<RES, JAC, WAVES, MAXB>(J, acc, racc, lruns, dc.nruns, lrows, dc.nr);
+    batch_stiffness<P, T, MAXB, RES, JAC, false>(mat, W, gref, acc, racc, lrows, lbase, dc.nr, gc, ok, el, idx);
+    lds_barrier();  // the part's accumulators are complete
 
+    dprev = dc;
     dc = dn;
     dn = dnn;
     dnn = d3;
@@ -1212,6 +1246,7 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
     gc = gn;
     gn = gnn;
   }
+  owner_write<RES, JAC, WAVES, MAXB>(J, acc, racc, lruns, dprev.nruns, lrows, dprev.nr);
 }
 
 // Streaming kernel for parts that take one batch per wavefront (the default partition), over one or
