@@ -864,25 +864,34 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
 }
 
 // Writes a part's accumulated BSR runs (one wave per run, one block per lane) and owned residual rows.
-template <bool RES, bool JAC, int WAVES, int MAXB>
-__device__ __forceinline__ void owner_write(const MeshJob &J, const double *acc, const double *racc,
-                                            const int4 *lruns, int nruns, const int *lrows, int nr) {
+// With ZERO, every value read is cleared, so the accumulators are all zero again afterwards (blocks past
+// the part's count were never touched).
+template <bool RES, bool JAC, int WAVES, int MAXB, bool ZERO = false>
+__device__ __forceinline__ void owner_write(const MeshJob &J, double *acc, double *racc, const int4 *lruns, int nruns,
+                                            const int *lrows, int nr) {
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   if (JAC && J.values) {
     const gptr_mut<double2> values = gmem_mut(reinterpret_cast<double2 *>(J.values));
     for (int r = wave; r < nruns; r += WAVES) {
       const int4 run = lruns[r];
-      const double *src = acc + run.x;
+      double *src = acc + run.x;
       const gptr_mut<double2> dst = values + 2 * int64_t(run.y);
       for (int i = lane; i < run.z; i += 64) {
         dst[2 * i] = make_double2(src[i], src[i + MAXB]);
         dst[2 * i + 1] = make_double2(src[i + 2 * MAXB], src[i + 3 * MAXB]);
+        if (ZERO) {
+#pragma unroll
+          for (int c = 0; c < 4; c++) src[i + c * MAXB] = 0;
+        }
       }
     }
   }
   if (RES && J.residual) {
     const gptr_mut<double2> res = gmem_mut(reinterpret_cast<double2 *>(J.residual));
-    for (int i = int(threadIdx.x); i < nr; i += WAVES * 64) res[lrows[i]] = reinterpret_cast<const double2 *>(racc)[i];
+    for (int i = int(threadIdx.x); i < nr; i += WAVES * 64) {
+      res[lrows[i]] = reinterpret_cast<const double2 *>(racc)[i];
+      if (ZERO) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
+    }
   }
 }
 
@@ -1205,6 +1214,13 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
   gather_level1<P>(J, dn.e0 + min(myv, dn.ne - 1), idx, gn);
   gather_level2<P>(J, idx, gc);
 
+  // The accumulators start zeroed; each write-out clears what it reads, so they are zero at every part's
+  // start. (The previous job's write-out finished before this call; the first barrier below orders it.)
+  lds_barrier();
+  if (JAC) zero_planes<THREADS, MAXB>(acc, MAXB);
+  if (RES) {
+    for (int i = tid; i < MAXR; i += THREADS) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
+  }
   PartDesc dprev = dc;
   for (int p = p0; p < p1; p++) {
     const bool ok = laneOk && myv < dc.ne;
@@ -1212,20 +1228,14 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
     // its stores drain while this part's geometry forms (both phases use only the wavefront's own LDS
     // besides the previous part's accumulators and tables, which nothing writes until the barrier below).
     batch_nodes<P, T>(W, gc, ok, el, idx);
-    if (p > p0) owner_write<RES, JAC, WAVES, MAXB>(J, acc, racc, lruns, dprev.nruns, lrows, dprev.nr);
+    if (p > p0) owner_write<RES, JAC, WAVES, MAXB, true>(J, acc, racc, lruns, dprev.nruns, lrows, dprev.nr);
     batch_geometry<P, T>(W, basis, mat, ok, el, idx);
     lds_barrier();  // the previous part's write-out has read the accumulators and tables
     if (tid < dc.nr) {
       lrows[tid] = tc.row;
       lbase[tid] = tc.lbase;
     }
-    if (JAC) {
-      if (tid < dc.nruns) lruns[tid] = make_int4(tc.runLds, tc.runGlobal, tc.runLen, 0);
-      zero_planes<THREADS, MAXB>(acc, dc.nb);
-    }
-    if (RES) {
-      for (int i = tid; i < dc.nr; i += THREADS) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
-    }
+    if (JAC && tid < dc.nruns) lruns[tid] = make_int4(tc.runLds, tc.runGlobal, tc.runLen, 0);
     // Prefetch (all loads of this step are issued here, before the stiffness phase).
     const PartDesc d3 = part_desc(J, min(p + 3, plast));
     Gather<P> gnn;
