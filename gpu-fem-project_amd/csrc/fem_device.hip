@@ -136,7 +136,7 @@ struct __align__(16) WaveLDS {
     float2 ns[O::EPW][4][O::N];             // MFMA stage: gradN_b at four quadrature points
   };
   double geo[O::EPW][O::GEO_STRIDE];  // per q: J^-1 (row-major), s_q, s_q*sigma_xx, s_q*sigma_yy, s_q*sigma_xy
-  int own[O::EPW][O::N];              // owner path: local row of node a in this part, or -1
+  int obase[O::EPW][O::N];            // owner path: LDS block base of node a's row in this part, or -1
   uint32_t rk[O::EPW][O::N][O::RPAD / 4];  // owner path: rk[el][b] byte a = slot of column b within row a
   unsigned char orow[O::EPW][O::N];        // owner path, orders >= 3: the element's owned nodes, compacted
 };
@@ -263,17 +263,20 @@ __device__ __forceinline__ void point_terms(const PointGeo &G, T &nx, T &ny, V2 
 
 // Residual entries of node b and (JAC) stiffness blocks for lane (el, b). K is symmetric, so each
 // unordered node pair {a, b} is formed once: lane b takes rows a = b + d (mod N), d = 0..N/2 (for even
-// N the pair at distance N/2 is taken by the lower lane only). emit(row, col, k) receives each block
-// K(row, col), row-major [xx, xy, yx, yy], in FP64, with a slot number: d for block (a, b) and D + d for
-// (b, a) = K(a, b)^T, emitted when a != b. Every lane of the wavefront must call this (it synchronises the wavefront);
-// `valid` masks the lane's LDS writes and emits.
+// N the pair at distance N/2 is taken by the lower lane only), and both K(a, b) and K(b, a) = K(a, b)^T
+// are emitted when a != b. where(row, col) gives the destination of block K(row, col) (negative: not
+// stored here); emit(at, k) adds block k, row-major [xx, xy, yx, yy] in FP64, at destination `at`. All
+// destinations are looked up before the first add, so the adds (which share the in-order wait counters
+// with the lookups) never wait on one another. Every lane of the wavefront must call this (it
+// synchronises the wavefront); `valid` masks the lane's LDS writes and emits.
 // Software pipeline over quadrature points: iteration q publishes point q+1's weighted gradients and
 // issues their reads, then accumulates point q's Gram sums from values read one iteration earlier, so
 // LDS latency overlaps the arithmetic. The two publication buffers alternate; a buffer is rewritten two
 // iterations after it was read, and the LDS executes one wavefront's accesses in order.
-template <int P, typename T, bool JAC, typename Emit>
+template <int P, typename T, bool JAC, typename Where, typename Emit>
 __device__ __forceinline__ void stage_stiffness(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el,
-                                                int b, bool valid, double &Rx, double &Ry, Emit &&emit) {
+                                                int b, bool valid, double &Rx, double &Ry, Where &&where,
+                                                Emit &&emit) {
   using O = Ord<P>;
   constexpr int N = O::N, Q = O::Q, H = O::H, D = O::D;
   using V2 = typename Vec2<T>::type;
@@ -348,6 +351,12 @@ __device__ __forceinline__ void stage_stiffness(WaveLDS<P, T> &W, const double2 
     accumulate(wA, nx, ny);
   }
   if (JAC && valid) {
+    int atA[D], atB[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      atA[d] = where(arow[d], b);
+      atB[d] = d != 0 ? where(b, arow[d]) : -1;
+    }
 #pragma unroll
     for (int d = 0; d < D; d++) {
       if (d == H && !lastPair) continue;
@@ -356,10 +365,10 @@ __device__ __forceinline__ void stage_stiffness(WaveLDS<P, T> &W, const double2 
       k[1] = double(lam * S[d][2] + mu * S[d][3]);
       k[2] = double(lam * S[d][3] + mu * S[d][2]);
       k[3] = double(c1 * S[d][1] + mu * S[d][0]);
-      emit(arow[d], b, d, k);
+      emit(atA[d], k);
       if (d != 0) {
         const double kt[4] = {k[0], k[2], k[1], k[3]};
-        emit(b, arow[d], D + d, kt);
+        emit(atB[d], kt);
       }
     }
   }
@@ -409,8 +418,9 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   wave_sync();
   const int *mel = args.map + int64_t(eid) * N * N;
   double Rx, Ry;
-  stage_stiffness<P, T, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, [&](int row, int col, int, const double(&k)[4]) __attribute__((always_inline)) {
-    double *blk = args.values + mel[row * N + col];
+  auto where = [&](int row, int col) __attribute__((always_inline)) { return mel[row * N + col]; };
+  stage_stiffness<P, T, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, where, [&](int at, const double(&k)[4]) __attribute__((always_inline)) {
+    double *blk = args.values + at;
     if (ATOMIC) {
 #pragma unroll
       for (int c = 0; c < 4; c++) unsafeAtomicAdd(blk + c, k[c]);
@@ -505,10 +515,10 @@ __device__ __forceinline__ void gather_level2(const MeshJob &J, int idx, Gather<
 // Software-pipelined over quadrature points like stage_stiffness: iteration q publishes point q+1's
 // weighted gradient of node b and issues the reads of the chunk's rows, then accumulates point q from
 // values read one iteration earlier.
-template <int P, typename T, bool JAC, int CH, typename Emit>
+template <int P, typename T, bool JAC, int CH, typename Where, typename Emit>
 __device__ __forceinline__ void rows_chunk(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el, int b,
                                            bool valid, int count, int base, bool first, double &Rx, double &Ry,
-                                           Emit &&emit) {
+                                           Where &&where, Emit &&emit) {
   using O = Ord<P>;
   constexpr int Q = O::Q;
   using V2 = typename Vec2<T>::type;
@@ -581,6 +591,9 @@ __device__ __forceinline__ void rows_chunk(WaveLDS<P, T> &W, const double2 *gref
     Ry += ry;
   }
   if (JAC && valid) {
+    int at[CH];
+#pragma unroll
+    for (int j = 0; j < CH; j++) at[j] = base + j < count ? where(arow[j], b) : -1;
 #pragma unroll
     for (int j = 0; j < CH; j++) {
       if (base + j >= count) break;
@@ -589,7 +602,7 @@ __device__ __forceinline__ void rows_chunk(WaveLDS<P, T> &W, const double2 *gref
       k[1] = double(lam * S[j][2] + mu * S[j][3]);
       k[2] = double(lam * S[j][3] + mu * S[j][2]);
       k[3] = double(c1 * S[j][1] + mu * S[j][0]);
-      emit(arow[j], b, -1, k);
+      emit(at[j], k);
     }
   }
   wave_sync();  // the next chunk republishes into the same buffers
@@ -599,18 +612,18 @@ __device__ __forceinline__ void rows_chunk(WaveLDS<P, T> &W, const double2 *gref
 // K(a, b) only for the rows a this part owns, W.orow[el][0..count), in chunks of 8 rows (4 in FP64; maxCount,
 // the largest count over the wavefront's elements, sets the number of chunks). Over all parts visiting an
 // element this forms each row of K_e once, instead of the whole symmetric K_e per visit. The residual of
-// node b is accumulated in the first chunk. emit(row, col, slot, k) as for stage_stiffness (slot -1).
-template <int P, typename T, bool JAC, typename Emit>
+// node b is accumulated in the first chunk. where and emit as for stage_stiffness.
+template <int P, typename T, bool JAC, typename Where, typename Emit>
 __device__ __forceinline__ void stage_stiffness_rows(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el,
                                                      int b, bool valid, int count, int maxCount, double &Rx,
-                                                     double &Ry, Emit &&emit) {
+                                                     double &Ry, Where &&where, Emit &&emit) {
   constexpr int CH = sizeof(T) == 8 ? 4 : 8;  // FP64 chunks are halved to stay within 256 registers
   Rx = 0;
   Ry = 0;
   const int chunks = JAC ? uniform(max((maxCount + CH - 1) / CH, 1)) : 1;
 #pragma unroll 1
   for (int c = 0; c < chunks; c++) {
-    rows_chunk<P, T, JAC, CH>(W, gref, m, el, b, valid, count, c * CH, c == 0, Rx, Ry, emit);
+    rows_chunk<P, T, JAC, CH>(W, gref, m, el, b, valid, count, c * CH, c == 0, Rx, Ry, where, emit);
   }
 }
 
@@ -628,8 +641,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // The residual of node b is accumulated in FP64 during the first chunk.
 template <int P, int MAXB, bool DET, typename WL>
 __device__ __forceinline__ void stage_stiffness_mfma(WL &W, const double2 *gref, const Material &m, int el, int b,
-                                                     unsigned long long ownMask, double &Rx, double &Ry, double *acc,
-                                                     const int *lbase) {
+                                                     unsigned long long ownMask, double &Rx, double &Ry, double *acc) {
   using O = Ord<P>;
   constexpr int N = O::N, Q = O::Q, EPW = O::EPW;
   constexpr int KSTEPS = (Q + 3) / 4, TILES = (2 * N + 15) / 16;
@@ -743,6 +755,19 @@ __device__ __forceinline__ void stage_stiffness_mfma(WL &W, const double2 *gref,
     wave_sync();  // the next chunk rewrites the slice
     const bool even = (lane & 1) == 0;
     const int jr = 2 * (lane / 16) + (even ? 0 : 1);  // chunk row this lane emits
+    // Destinations of this lane's blocks, all looked up before the first add.
+    int at[EPW][TILES];
+#pragma unroll
+    for (int e = 0; e < EPW; e++) {
+      const bool rowOk = c * 8 + jr < cnt[e];
+      const int arow = W.orow[e][rowOk ? c * 8 + jr : 0];
+#pragma unroll
+      for (int t = 0; t < TILES; t++) {
+        const int bcol = (16 * t + lane % 16) / 2;
+        const int at0 = W.obase[e][arow] + rank_of(W, e, arow, bcol < N ? bcol : 0);
+        at[e][t] = rowOk && bcol < N ? at0 : -1;
+      }
+    }
 #pragma unroll
     for (int e = 0; e < EPW; e++) {
 #pragma unroll
@@ -753,11 +778,8 @@ __device__ __forceinline__ void stage_stiffness_mfma(WL &W, const double2 *gref,
         for (int r = 0; r < 4; r++) o[r] = __shfl_xor(d[r], 1);
         const float sxx = even ? d[0] : o[2], syx = even ? d[1] : o[3];
         const float sxy = even ? o[0] : d[2], syy = even ? o[1] : d[3];
-        const int bcol = (16 * t + lane % 16) / 2;
-        if (bcol < N && c * 8 + jr < cnt[e]) {
-          const int arow = W.orow[e][c * 8 + jr];
-          const int ownR = W.own[e][arow];
-          double *dst = acc + (lbase[ownR] + rank_of(W, e, arow, bcol));
+        if (at[e][t] >= 0) {
+          double *dst = acc + at[e][t];
           const double k[4] = {double(c1 * sxx + mu * syy), double(lam * sxy + mu * syx),
                                double(lam * syx + mu * sxy), double(c1 * syy + mu * sxx)};
           if (DET) {
@@ -805,7 +827,7 @@ __device__ __forceinline__ void batch_stiffness(const Material &mat, WaveLDS<P, 
   int own = -1;
   if (valid) {
     own = (g.lrow < nr && lrows[g.lrow] == g.node) ? g.lrow : -1;
-    W.own[el][idx] = own;
+    W.obase[el][idx] = own >= 0 ? lbase[own] : -1;
   }
   // Orders >= 3: compact this element's owned rows (lanes of element e are e*N .. e*N+N-1).
   int count = 0, maxCount = 0;
@@ -822,10 +844,14 @@ __device__ __forceinline__ void batch_stiffness(const Material &mat, WaveLDS<P, 
   }
   wave_sync();
   double Rx, Ry;
-  auto emit = [&](int row, int col, int, const double(&k)[4]) __attribute__((always_inline)) {
-    const int ownR = W.own[el][row];
-    if (ownR < 0) return;
-    double *dst = acc + (lbase[ownR] + rank_of(W, el, row, col));
+  auto where = [&](int row, int col) __attribute__((always_inline)) {
+    const int ob = W.obase[el][row];
+    const int rk = rank_of(W, el, row, col);  // read unconditionally: no branch between the lookups
+    return ob >= 0 ? ob + rk : -1;
+  };
+  auto emit = [&](int at, const double(&k)[4]) __attribute__((always_inline)) {
+    if (at < 0) return;
+    double *dst = acc + at;
     if (DET) {
 #pragma unroll
       for (int c = 0; c < 4; c++) dst[c * MAXB] += k[c];
@@ -835,11 +861,11 @@ __device__ __forceinline__ void batch_stiffness(const Material &mat, WaveLDS<P, 
     }
   };
   if constexpr (P >= 3 && JAC && std::is_same<T, float>::value) {
-    stage_stiffness_mfma<P, MAXB, DET>(W, gref, mat, el, idx, ownMask, Rx, Ry, acc, lbase);
+    stage_stiffness_mfma<P, MAXB, DET>(W, gref, mat, el, idx, ownMask, Rx, Ry, acc);
   } else if constexpr (P >= 3) {
-    stage_stiffness_rows<P, T, JAC>(W, gref, mat, el, idx, valid, count, maxCount, Rx, Ry, emit);
+    stage_stiffness_rows<P, T, JAC>(W, gref, mat, el, idx, valid, count, maxCount, Rx, Ry, where, emit);
   } else {
-    stage_stiffness<P, T, JAC>(W, gref, mat, el, idx, valid, Rx, Ry, emit);
+    stage_stiffness<P, T, JAC>(W, gref, mat, el, idx, valid, Rx, Ry, where, emit);
   }
   if (RES && valid && own >= 0) {
     if (DET) {
@@ -866,17 +892,25 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
 // Writes a part's accumulated BSR runs (one wave per run, one block per lane) and owned residual rows.
 // With ZERO, every value read is cleared, so the accumulators are all zero again afterwards (blocks past
 // the part's count were never touched).
-template <bool RES, bool JAC, int WAVES, int MAXB, bool ZERO = false>
-__device__ __forceinline__ void owner_write(const MeshJob &J, double *acc, double *racc, const int4 *lruns, int nruns,
+// A part's write run: LDS block start, global block start, blocks. In LDS either as int4 (x, y, z) or
+// packed as int2 (x = LDS start | blocks << 16, y = global start).
+struct Run {
+  int lds, global, len;
+};
+__device__ __forceinline__ Run unpack_run(const int4 &r) { return Run{r.x, r.y, r.z}; }
+__device__ __forceinline__ Run unpack_run(const int2 &r) { return Run{r.x & 0xffff, r.y, r.x >> 16}; }
+
+template <bool RES, bool JAC, int WAVES, int MAXB, bool ZERO = false, typename RunT>
+__device__ __forceinline__ void owner_write(const MeshJob &J, double *acc, double *racc, const RunT *lruns, int nruns,
                                             const int *lrows, int nr) {
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   if (JAC && J.values) {
     const gptr_mut<double2> values = gmem_mut(reinterpret_cast<double2 *>(J.values));
     for (int r = wave; r < nruns; r += WAVES) {
-      const int4 run = lruns[r];
-      double *src = acc + run.x;
-      const gptr_mut<double2> dst = values + 2 * int64_t(run.y);
-      for (int i = lane; i < run.z; i += 64) {
+      const Run run = unpack_run(lruns[r]);
+      double *src = acc + run.lds;
+      const gptr_mut<double2> dst = values + 2 * int64_t(run.global);
+      for (int i = lane; i < run.len; i += 64) {
         dst[2 * i] = make_double2(src[i], src[i + MAXB]);
         dst[2 * i + 1] = make_double2(src[i + 2 * MAXB], src[i + 3 * MAXB]);
         if (ZERO) {
@@ -1186,15 +1220,17 @@ __device__ __forceinline__ void load_part_tables(const MeshJob &J, const PartDes
   t.runLen = r.z;
 }
 
-// Parts [p0, p1) of one job, software-pipelined. A step stages part p's nodes, writes out part p-1 and forms
-// part p's geometry, then (after a barrier) sets up part p's tables and accumulators and issues part p+1's
-// coordinate/state/rank gathers (its node ids arrived during the previous step), part p+2's node ids, part
-// p+1's row tables and part p+3's descriptor, and computes part p's stiffness from registers and LDS only.
-// The barriers wait for LDS traffic only, so no wait on HBM falls inside the compute phases.
+// Parts [p0, p1) of one job, software-pipelined. The row and run tables are double-buffered by part parity,
+// so a step needs two barriers. Step p stores part p's tables, stages its nodes (gathered during the
+// previous step), writes out part p-1 (its stores drain while the geometry forms) and forms part p's
+// geometry; after barrier A it issues part p+1's coordinate/state/rank gathers (their node ids arrived
+// during the previous step), part p+2's node ids, part p+1's tables and part p+3's descriptor, and computes
+// part p's stiffness from registers and LDS only; barrier C then closes part p's accumulators. The barriers
+// wait for LDS traffic only, so no wait on HBM falls inside the compute phases.
 template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC>
 __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, WaveLDS<P, T> &W, const Basis1D &basis,
-                                             const double2 *gref, double *acc, double *racc, int *lrows, int *lbase,
-                                             int4 *lruns) {
+                                             const double2 *gref, double *acc, double *racc, int (*lrows)[MAXR],
+                                             int *lbase, int2 (*lruns)[MAXR]) {
   using O = Ord<P>;
   constexpr int THREADS = WAVES * 64;
   const int tid = threadIdx.x, lane = tid % 64;
@@ -1223,19 +1259,24 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
   }
   PartDesc dprev = dc;
   for (int p = p0; p < p1; p++) {
+    const int tb = (p - p0) & 1;  // table buffer of part p; part p-1's is tb ^ 1
     const bool ok = laneOk && myv < dc.ne;
-    // Stage this part's nodes (gathered during the previous step), then write out the previous part:
-    // its stores drain while this part's geometry forms (both phases use only the wavefront's own LDS
-    // besides the previous part's accumulators and tables, which nothing writes until the barrier below).
-    batch_nodes<P, T>(W, gc, ok, el, idx);
-    if (p > p0) owner_write<RES, JAC, WAVES, MAXB, true>(J, acc, racc, lruns, dprev.nruns, lrows, dprev.nr);
-    batch_geometry<P, T>(W, basis, mat, ok, el, idx);
-    lds_barrier();  // the previous part's write-out has read the accumulators and tables
+    // Part p's tables: their buffer was last read by part p-2's write-out, and lbase by part p-1's
+    // stiffness, both before the previous step's closing barrier.
     if (tid < dc.nr) {
-      lrows[tid] = tc.row;
+      lrows[tb][tid] = tc.row;
       lbase[tid] = tc.lbase;
     }
-    if (JAC && tid < dc.nruns) lruns[tid] = make_int4(tc.runLds, tc.runGlobal, tc.runLen, 0);
+    if (JAC && tid < dc.nruns) lruns[tb][tid] = make_int2(tc.runLds | (tc.runLen << 16), tc.runGlobal);
+    // Stage this part's nodes, then write out the previous part: its stores drain while this part's
+    // geometry forms (both phases use only the wavefront's own LDS besides the previous part's
+    // accumulators and tables).
+    batch_nodes<P, T>(W, gc, ok, el, idx);
+    if (p > p0) {
+      owner_write<RES, JAC, WAVES, MAXB, true>(J, acc, racc, lruns[tb ^ 1], dprev.nruns, lrows[tb ^ 1], dprev.nr);
+    }
+    batch_geometry<P, T>(W, basis, mat, ok, el, idx);
+    lds_barrier();  // A: part p's tables are stored; part p-1's accumulators are written out and cleared
     // Prefetch (all loads of this step are issued here, before the stiffness phase).
     const PartDesc d3 = part_desc(J, min(p + 3, plast));
     Gather<P> gnn;
@@ -1243,10 +1284,9 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
     PartTables tn;
     load_part_tables(J, dn, tid, tn);
     gather_level1<P>(J, dnn.e0 + min(myv, dnn.ne - 1), idx, gnn);
-    lds_barrier();
 
-    batch_stiffness<P, T, MAXB, RES, JAC, false>(mat, W, gref, acc, racc, lrows, lbase, dc.nr, gc, ok, el, idx);
-    lds_barrier();  // the part's accumulators are complete
+    batch_stiffness<P, T, MAXB, RES, JAC, false>(mat, W, gref, acc, racc, lrows[tb], lbase, dc.nr, gc, ok, el, idx);
+    lds_barrier();  // C: the part's accumulators are complete
 
     dprev = dc;
     dc = dn;
@@ -1256,7 +1296,8 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
     gc = gn;
     gn = gnn;
   }
-  owner_write<RES, JAC, WAVES, MAXB>(J, acc, racc, lruns, dprev.nruns, lrows, dprev.nr);
+  const int tl = (p1 - 1 - p0) & 1;
+  owner_write<RES, JAC, WAVES, MAXB>(J, acc, racc, lruns[tl], dprev.nruns, lrows[tl], dprev.nr);
 }
 
 // Streaming kernel for parts that take one batch per wavefront (the default partition), over one or
@@ -1270,14 +1311,15 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(FEM_
 void owner_stream_kernel(const OwnerArgs args) {
   using O = Ord<P>;
   static_assert(MAXR <= WAVES * 64, "one owned row per thread");
+  static_assert(MAXB < 32768, "packed runs hold LDS starts and lengths in 16 bits");
   __shared__ WaveLDS<P, T> wl[WAVES];
   __shared__ Basis1D basis;
   __shared__ double2 gref[O::Q * O::N];
   __shared__ __align__(16) double acc[JAC ? 4 * MAXB : 2];
   __shared__ __align__(16) double racc[RES ? 2 * MAXR : 2];
-  __shared__ int lrows[MAXR];
+  __shared__ int lrows[2][MAXR];
   __shared__ int lbase[MAXR];
-  __shared__ int4 lruns[JAC ? MAXR : 1];
+  __shared__ int2 lruns[JAC ? 2 : 1][MAXR];
 
   // Contiguous range of global parts per workgroup; workgroup w runs on XCD w % 8, and each XCD's
   // workgroups share one contiguous (spatially adjacent, Morton-ordered) stretch of parts.

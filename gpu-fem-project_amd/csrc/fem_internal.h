@@ -60,7 +60,7 @@ constexpr OwnerConfig owner_config(int order) {
 // (1.53 visits per element at 1000x1000 order 2, against 1.83). Orders 3-4: two workgroups of 4 waves
 // per CU, each with half the LDS.
 constexpr OwnerConfig owner_config(int order) {
-  return order == 1   ? OwnerConfig{8, 2720, 384, 128, 0}
+  return order == 1   ? OwnerConfig{8, 2720, 352, 128, 0}
          : order == 2 ? OwnerConfig{8, 2720, 320, 56, 0}
          : order == 3 ? OwnerConfig{4, 1312, 96, 16, 0}
                       : OwnerConfig{4, 1312, 64, 8, 0};
