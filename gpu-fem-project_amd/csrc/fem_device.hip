@@ -104,6 +104,18 @@ __device__ __forceinline__ gptr_mut<T> gmem_mut(T *p) {
   return (gptr_mut<T>)p;
 }
 
+// Streaming (non-temporal) global accesses, for data touched once per assembly: the plan's visit tables and
+// column ranks, and the BSR values written out. They are marked so that the L2 favours the node
+// coordinates and states, which neighbouring parts gather again.
+typedef double double2v __attribute__((ext_vector_type(2)));
+template <typename T>
+__device__ __forceinline__ T load_once(gptr<T> p) {
+  return __builtin_nontemporal_load(p);
+}
+__device__ __forceinline__ void store_once(gptr_mut<double2> p, double a, double b) {
+  __builtin_nontemporal_store(double2v{a, b}, reinterpret_cast<FEM_GLOBAL_AS double2v *>(p));
+}
+
 // Workgroup barrier for LDS traffic only: waits for this wave's LDS operations, then s_barrier. Unlike
 // __syncthreads() it does not wait for outstanding global loads and stores, so prefetches stay in flight
 // across it. The memory clobber keeps the compiler from moving LDS accesses across it.
@@ -494,8 +506,8 @@ struct Gather {
 // Level 1: node id and element id of visit `visit` (always in bounds: callers clamp the visit).
 template <int P>
 __device__ __forceinline__ void gather_level1(const MeshJob &J, int visit, int idx, Gather<P> &g) {
-  g.node = gmem(J.visitConn)[int64_t(visit) * Ord<P>::N + idx];
-  g.eid = gmem(J.visitEid)[visit];
+  g.node = load_once(gmem(J.visitConn) + int64_t(visit) * Ord<P>::N + idx);
+  g.eid = load_once(gmem(J.visitEid) + visit);
 }
 
 // Level 2: coordinates, states, owner row and column ranks of the node/element of level 1.
@@ -507,7 +519,7 @@ __device__ __forceinline__ void gather_level2(const MeshJob &J, int idx, Gather<
   constexpr int W = Ord<P>::RPAD / 4;
   const gptr<uint32_t> r = gmem(J.rank) + (int64_t(g.eid) * Ord<P>::N + idx) * W;
 #pragma unroll
-  for (int i = 0; i < W; i++) g.rk[i] = r[i];
+  for (int i = 0; i < W; i++) g.rk[i] = load_once(r + i);
 }
 
 // One chunk of the row-restricted stiffness: rows W.orow[el][base .. base+CH) (clamped to the element's
@@ -911,8 +923,8 @@ __device__ __forceinline__ void owner_write(const MeshJob &J, double *acc, doubl
       double *src = acc + run.lds;
       const gptr_mut<double2> dst = values + 2 * int64_t(run.global);
       for (int i = lane; i < run.len; i += 64) {
-        dst[2 * i] = make_double2(src[i], src[i + MAXB]);
-        dst[2 * i + 1] = make_double2(src[i + 2 * MAXB], src[i + 3 * MAXB]);
+        store_once(dst + 2 * i, src[i], src[i + MAXB]);
+        store_once(dst + 2 * i + 1, src[i + 2 * MAXB], src[i + 3 * MAXB]);
         if (ZERO) {
 #pragma unroll
           for (int c = 0; c < 4; c++) src[i + c * MAXB] = 0;
