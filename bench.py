@@ -486,7 +486,10 @@ def main():
     ap.add_argument("--requests", type=int, default=10000, help="serve-poisson: requests at the main load")
     ap.add_argument("--side-requests", type=int, default=2000, help="serve-poisson: requests at 0.5 and 0.95 load")
     ap.add_argument("--pool", type=int, default=0, help="override the mesh-pool size (0: the workload's own)")
+    ap.add_argument("--autotune", action="store_true", help="plan path: build the plan with FEM_PLAN_AUTOTUNE")
     args = ap.parse_args()
+    if args.autotune and args.path != "plan":
+        ap.error("--autotune applies to --path plan only")
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X, and the assembly path is sized for "
               "one GPU's share (multi-GPU sharding is described, not run). Use --gpus 1.", file=sys.stderr)
@@ -516,7 +519,10 @@ def main():
     plan = None
     if args.path.startswith("plan"):
         t0 = time.time()
-        plan = F.Plan(mesh, pat, flags=F.FEM_PLAN_DETERMINISTIC if args.path == "plan-det" else 0)
+        flags = F.FEM_PLAN_DETERMINISTIC if args.path == "plan-det" else 0
+        if args.autotune:
+            flags |= F.FEM_PLAN_AUTOTUNE
+        plan = F.Plan(mesh, pat, flags=flags)
         info = plan.info()
         log(f"plan built in {time.time() - t0:.1f} s: {info}")
 
@@ -580,7 +586,7 @@ def main():
         cpu = cpu_baseline(mesh, u, bmap, pat.num_blocks, args.cpu_seconds)
         log(f"cpu baseline {cpu['value']:.4g} elements/s on {cpu['cores']} threads")
 
-    traffic = load_traffic(args.workload, args.path)
+    traffic = None if args.autotune else load_traffic(args.workload, args.path)
     out = {
         "metric": "res+Jac elements/s & % HBM roofline",
         "value": value,
@@ -596,7 +602,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]['desc']}",
                    "elements": mesh.num_elements, "nodes": mesh.num_nodes, "bsr_blocks": pat.num_blocks,
-                   "path": args.path, "precision": args.precision, "parallelism": "single GPU"},
+                   "path": args.path + ("+autotune" if args.autotune else ""), "precision": args.precision,
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": ("owner_kernel" if args.path == "plan-det" else "owner_stream_kernel") if plan is not None

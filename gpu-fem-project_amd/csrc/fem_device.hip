@@ -124,6 +124,16 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // Wave-uniform value (keeps part- and job-level indices in scalar registers).
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Distance in doubles between the four component planes of a part accumulator of MAXB blocks. The 4-double
+// pad puts planes 0 and 2 (and 1 and 3) 64 bytes apart modulo 128, so the write-out's paired reads of
+// block halves (xx, xy from planes 0-1 by even lanes, yx, yy from planes 2-3 by odd lanes) fall in
+// different banks.
+template <int MAXB>
+constexpr int acc_plane() {
+  static_assert(MAXB % 8 == 0, "plane offsets stay 16-byte aligned and 64 bytes apart modulo 128");
+  return MAXB + 4;
+}
+
 template <typename T>
 struct Vec2;
 template <>
@@ -796,10 +806,10 @@ __device__ __forceinline__ void stage_stiffness_mfma(WL &W, const double2 *gref,
                                double(lam * syx + mu * sxy), double(c1 * syy + mu * sxx)};
           if (DET) {
 #pragma unroll
-            for (int cc = 0; cc < 4; cc++) dst[cc * MAXB] += k[cc];
+            for (int cc = 0; cc < 4; cc++) dst[cc * acc_plane<MAXB>()] += k[cc];
           } else {
 #pragma unroll
-            for (int cc = 0; cc < 4; cc++) atomicAdd(dst + cc * MAXB, k[cc]);
+            for (int cc = 0; cc < 4; cc++) atomicAdd(dst + cc * acc_plane<MAXB>(), k[cc]);
           }
         }
       }
@@ -807,7 +817,7 @@ __device__ __forceinline__ void stage_stiffness_mfma(WL &W, const double2 *gref,
   }
 }
 
-// The accumulator holds a part's BSR blocks as four planes, acc[c * MAXB + block] for block component c
+// The accumulator holds a part's BSR blocks as four planes, acc[c * acc_plane + block] for block component c
 // (row-major xx, xy, yx, yy): the lanes of one LDS atomic then spread over all banks.
 //
 // An element batch runs in three phases: batch_nodes stages the gathered nodes and column ranks,
@@ -866,10 +876,10 @@ __device__ __forceinline__ void batch_stiffness(const Material &mat, WaveLDS<P, 
     double *dst = acc + at;
     if (DET) {
 #pragma unroll
-      for (int c = 0; c < 4; c++) dst[c * MAXB] += k[c];
+      for (int c = 0; c < 4; c++) dst[c * acc_plane<MAXB>()] += k[c];
     } else {
 #pragma unroll
-      for (int c = 0; c < 4; c++) atomicAdd(dst + c * MAXB, k[c]);
+      for (int c = 0; c < 4; c++) atomicAdd(dst + c * acc_plane<MAXB>(), k[c]);
     }
   };
   if constexpr (P >= 3 && JAC && std::is_same<T, float>::value) {
@@ -901,9 +911,6 @@ __device__ __forceinline__ void owner_batch(const Material &mat, WaveLDS<P, T> &
   batch_stiffness<P, T, MAXB, RES, JAC, DET>(mat, W, gref, acc, racc, lrows, lbase, nr, g, valid, el, idx);
 }
 
-// Writes a part's accumulated BSR runs (one wave per run, one block per lane) and owned residual rows.
-// With ZERO, every value read is cleared, so the accumulators are all zero again afterwards (blocks past
-// the part's count were never touched).
 // A part's write run: LDS block start, global block start, blocks. In LDS either as int4 (x, y, z) or
 // packed as int2 (x = LDS start | blocks << 16, y = global start).
 struct Run {
@@ -912,7 +919,12 @@ struct Run {
 __device__ __forceinline__ Run unpack_run(const int4 &r) { return Run{r.x, r.y, r.z}; }
 __device__ __forceinline__ Run unpack_run(const int2 &r) { return Run{r.x & 0xffff, r.y, r.x >> 16}; }
 
-template <bool RES, bool JAC, int WAVES, int MAXB, bool ZERO = false, typename RunT>
+// Writes a part's accumulated BSR runs (one wave per run) and owned residual rows. With HALVES, lane pairs
+// move the two 16-byte halves of a block (contiguous store instructions; faster at orders 1-2), else each
+// lane moves a whole block (faster at orders 3-4, whose write-out loops are shorter). With ZERO, every
+// value read is cleared, so the accumulators are all zero again afterwards (blocks past the part's count
+// were never touched).
+template <bool RES, bool JAC, int WAVES, int MAXB, bool ZERO = false, bool HALVES = false, typename RunT>
 __device__ __forceinline__ void owner_write(const MeshJob &J, double *acc, double *racc, const RunT *lruns, int nruns,
                                             const int *lrows, int nr) {
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -920,14 +932,28 @@ __device__ __forceinline__ void owner_write(const MeshJob &J, double *acc, doubl
     const gptr_mut<double2> values = gmem_mut(reinterpret_cast<double2 *>(J.values));
     for (int r = wave; r < nruns; r += WAVES) {
       const Run run = unpack_run(lruns[r]);
-      double *src = acc + run.lds;
       const gptr_mut<double2> dst = values + 2 * int64_t(run.global);
-      for (int i = lane; i < run.len; i += 64) {
-        store_once(dst + 2 * i, src[i], src[i + MAXB]);
-        store_once(dst + 2 * i + 1, src[i + 2 * MAXB], src[i + 3 * MAXB]);
-        if (ZERO) {
+      if (HALVES) {
+        // Lane pair (2k, 2k+1) moves the two 16-byte halves of block k, so each store instruction writes
+        // 1 KB contiguously.
+        for (int i = lane; i < 2 * run.len; i += 64) {
+          double *src = acc + 2 * (i & 1) * acc_plane<MAXB>() + run.lds + (i >> 1);
+          store_once(dst + i, src[0], src[acc_plane<MAXB>()]);
+          if (ZERO) {
+            src[0] = 0;
+            src[acc_plane<MAXB>()] = 0;
+          }
+        }
+      } else {
+        // One block per lane (two 16-byte stores).
+        double *src = acc + run.lds;
+        for (int i = lane; i < run.len; i += 64) {
+          store_once(dst + 2 * i, src[i], src[i + acc_plane<MAXB>()]);
+          store_once(dst + 2 * i + 1, src[i + 2 * acc_plane<MAXB>()], src[i + 3 * acc_plane<MAXB>()]);
+          if (ZERO) {
 #pragma unroll
-          for (int c = 0; c < 4; c++) src[i + c * MAXB] = 0;
+            for (int c = 0; c < 4; c++) src[i + c * acc_plane<MAXB>()] = 0;
+          }
         }
       }
     }
@@ -947,7 +973,7 @@ __device__ __forceinline__ void zero_planes(double *acc, int nb) {
   const int h = (nb + 1) / 2;  // MAXB is even, so the rounded-up pair stays inside the plane
 #pragma unroll
   for (int c = 0; c < 4; c++) {
-    double2 *plane = reinterpret_cast<double2 *>(acc + c * MAXB);
+    double2 *plane = reinterpret_cast<double2 *>(acc + c * acc_plane<MAXB>());
     for (int i = int(threadIdx.x); i < h; i += THREADS) plane[i] = make_double2(0, 0);
   }
 }
@@ -1106,10 +1132,10 @@ __device__ __forceinline__ void owner_batch_fat(const Material &mat, FatWaveLDS<
     double *dst = acc + o;
     if (DET) {
 #pragma unroll
-      for (int c = 0; c < 4; c++) dst[c * MAXB] += k[c];
+      for (int c = 0; c < 4; c++) dst[c * acc_plane<MAXB>()] += k[c];
     } else {
 #pragma unroll
-      for (int c = 0; c < 4; c++) atomicAdd(dst + c * MAXB, k[c]);
+      for (int c = 0; c < 4; c++) atomicAdd(dst + c * acc_plane<MAXB>(), k[c]);
     }
   });
   if (RES && valid && own >= 0) {
@@ -1136,7 +1162,7 @@ __global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args)
   __shared__ WLT wl[WAVES];
   __shared__ Basis1D basis;
   __shared__ double2 gref[FAT ? 1 : O::Q * O::N];
-  __shared__ __align__(16) double acc[JAC ? 4 * MAXB : 2];
+  __shared__ __align__(16) double acc[JAC ? 4 * acc_plane<MAXB>() : 2];
   __shared__ __align__(16) double racc[RES ? 2 * MAXR : 2];
   __shared__ int lrows[MAXR];
   __shared__ int lbase[MAXR];
@@ -1213,7 +1239,7 @@ __global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args)
     }
     __syncthreads();
   }
-  owner_write<RES, JAC, WAVES, MAXB>(J, acc, racc, lruns, d.nruns, lrows, d.nr);
+  owner_write<RES, JAC, WAVES, MAXB, false, (P <= 2)>(J, acc, racc, lruns, d.nruns, lrows, d.nr);
 }
 
 // One thread's entries of a part's row tables: owned row `tid` (node, LDS block base) and run `tid`.
@@ -1285,7 +1311,8 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
     // accumulators and tables).
     batch_nodes<P, T>(W, gc, ok, el, idx);
     if (p > p0) {
-      owner_write<RES, JAC, WAVES, MAXB, true>(J, acc, racc, lruns[tb ^ 1], dprev.nruns, lrows[tb ^ 1], dprev.nr);
+      owner_write<RES, JAC, WAVES, MAXB, true, (P <= 2)>(J, acc, racc, lruns[tb ^ 1], dprev.nruns, lrows[tb ^ 1],
+                                                         dprev.nr);
     }
     batch_geometry<P, T>(W, basis, mat, ok, el, idx);
     lds_barrier();  // A: part p's tables are stored; part p-1's accumulators are written out and cleared
@@ -1309,7 +1336,7 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
     gn = gnn;
   }
   const int tl = (p1 - 1 - p0) & 1;
-  owner_write<RES, JAC, WAVES, MAXB>(J, acc, racc, lruns[tl], dprev.nruns, lrows[tl], dprev.nr);
+  owner_write<RES, JAC, WAVES, MAXB, false, (P <= 2)>(J, acc, racc, lruns[tl], dprev.nruns, lrows[tl], dprev.nr);
 }
 
 // Streaming kernel for parts that take one batch per wavefront (the default partition), over one or
@@ -1327,7 +1354,7 @@ void owner_stream_kernel(const OwnerArgs args) {
   __shared__ WaveLDS<P, T> wl[WAVES];
   __shared__ Basis1D basis;
   __shared__ double2 gref[O::Q * O::N];
-  __shared__ __align__(16) double acc[JAC ? 4 * MAXB : 2];
+  __shared__ __align__(16) double acc[JAC ? 4 * acc_plane<MAXB>() : 2];
   __shared__ __align__(16) double racc[RES ? 2 * MAXR : 2];
   __shared__ int lrows[2][MAXR];
   __shared__ int lbase[MAXR];
@@ -1452,7 +1479,7 @@ __device__ __forceinline__ void fat_parts(const MeshJob &J, int p0, int p1, FatW
       continue;
     }
     lds_barrier();
-    owner_write<RES, JAC, WAVES, MAXB>(J, acc, racc, lruns, dc.nruns, lrows, dc.nr);
+    owner_write<RES, JAC, WAVES, MAXB, false, (P <= 2)>(J, acc, racc, lruns, dc.nruns, lrows, dc.nr);
     if (p == plast) break;
     p++;
     dc = dn;
@@ -1470,7 +1497,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(1, 1
   using O = Ord<P>;
   __shared__ FatWaveLDS<P, T> wl[WAVES];
   __shared__ Basis1D basis;
-  __shared__ __align__(16) double acc[JAC ? 4 * MAXB : 2];
+  __shared__ __align__(16) double acc[JAC ? 4 * acc_plane<MAXB>() : 2];
   __shared__ __align__(16) double racc[RES ? 2 * MAXR : 2];
   __shared__ int lrows[MAXR];
   __shared__ int lbase[MAXR];
