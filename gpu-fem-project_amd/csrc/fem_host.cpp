@@ -140,6 +140,15 @@ void row_columns(int n, int npe, const int *connPtr, const int *conn, const std:
 }
 
 // Position of `col` in cols[b, e) (sorted); -1 if absent.
+// Bijective 32-bit mix (xor-shift-multiply), used as a fixed pseudo-random sort key.
+inline uint32_t scramble(uint32_t v) {
+  v ^= v >> 16;
+  v *= 0x7feb352du;
+  v ^= v >> 15;
+  v *= 0x846ca68bu;
+  return v ^ (v >> 16);
+}
+
 inline int find_sorted(const int *cols, int b, int e, int col) {
   const int *p = std::lower_bound(cols + b, cols + e, col);
   return (p != cols + e && *p == col) ? int(p - cols) : -1;
@@ -343,7 +352,10 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
   if (rc) return rc;
   out.numColours = numColours;
 
-  // Elements touching each part, grouped by colour then id (deterministic plans sum colour by colour).
+  // Elements touching each part, grouped by colour (deterministic plans sum colour by colour) and in a
+  // fixed pseudo-random order within a colour. A wavefront takes consecutive elements of this order, and
+  // their LDS adds then land on less regular bank patterns than for neighbouring ids of a structured mesh
+  // (which repeat the same column ranks at regular row offsets).
   std::vector<std::vector<int>> partElems(P);
   parallel_for(P, [&](int64_t b, int64_t e) {
     std::vector<int> tmp;
@@ -355,7 +367,9 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
       }
       std::sort(tmp.begin(), tmp.end());
       tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-      std::stable_sort(tmp.begin(), tmp.end(), [&](int x, int y) { return colour[x] < colour[y]; });
+      std::sort(tmp.begin(), tmp.end(), [&](int x, int y) {
+        return colour[x] != colour[y] ? colour[x] < colour[y] : scramble(uint32_t(x)) < scramble(uint32_t(y));
+      });
       partElems[p] = tmp;
     }
   });
