@@ -250,7 +250,8 @@ def bench_batch(args, F, torch):
         "dtype": "f64" if prec == F.FEM_PREC_FP64 else "f32+f64acc",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {POOL_WORKLOADS[args.workload]}", "meshes": count,
-                   "elements": total_el, "orders": by_order, "graph": graph, "parallelism": "single GPU"},
+                   "elements": total_el, "orders": by_order, "graph": graph, "precision": args.precision,
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": b_alg / t_batch / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": b_alg / t_batch / 1e9 / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": b_alg},
         "baselines": {"one_plan_launch_per_mesh_ms": t_plans * 1e3, "plain_fp64_atomic_per_mesh_ms": t_plain * 1e3,
@@ -479,7 +480,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="square1000-p2", choices=sorted(WORKLOADS) + sorted(POOL_WORKLOADS))
     ap.add_argument("--path", default="plan", choices=["plan", "plan-det", "atomic"])
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
+                    help="default: fp64 for single meshes, mixed for batch256-mixed")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -490,6 +492,8 @@ def main():
     args = ap.parse_args()
     if args.autotune and args.path != "plan":
         ap.error("--autotune applies to --path plan only")
+    if args.precision is None:
+        args.precision = "mixed" if args.workload == "batch256-mixed" else "fp64"
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X, and the assembly path is sized for "
               "one GPU's share (multi-GPU sharding is described, not run). Use --gpus 1.", file=sys.stderr)
