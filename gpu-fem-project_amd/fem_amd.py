@@ -62,6 +62,8 @@ SIGNATURES = {
     "fem_bsr_pattern_rowp": ([_I, _I, _I, _P, _P, _P], _I),
     "fem_bsr_pattern_cols": ([_I, _I, _I, _P, _P, _P, _P], _I),
     "fem_element_bcsr_map": ([_P, _P, _I, _I, _I, _P, _P, _I, _P], _I),
+    "fem_bsr_pattern_device": ([_I, _I, _I, _P, _P, _P, _P, _I64, _PI64, _P], _I),
+    "fem_element_bcsr_map_device": ([_P, _P, _I, _I, _I, _P, _P, _I, _P, _P], _I),
     "fem_element_colouring": ([_I, _I, _I, _P, _P, _P, ctypes.POINTER(_I)], _I),
     "fem_residual": ([_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _I, _P, ctypes.POINTER(_D)], _I),
     "fem_jacobian": ([_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _P, _P, _I, _P, ctypes.POINTER(_D)], _I),
@@ -265,6 +267,32 @@ def element_bcsr_map(mesh: Mesh, pattern: Pattern, bsize: int = 2) -> np.ndarray
     _check(load().fem_element_bcsr_map(_ptr(cp), _ptr(mesh.conn), mesh.num_elements, mesh.npe, mesh.num_nodes,
                                        _ptr(pattern.rowp), _ptr(pattern.cols), bsize, _ptr(out)))
     return out
+
+
+def bsr_pattern_device(num_nodes, num_elements, npe, conn_ptr, conn, stream=None):
+    """Device-built pattern from device arrays (torch int32 tensors): returns (rowp, cols) torch tensors on the
+    same device, identical to bsr_pattern's arrays."""
+    import torch
+
+    rowp = torch.empty(num_nodes + 1, dtype=torch.int32, device=conn.device)
+    nb = ctypes.c_int64(0)
+    lib = load()
+    _check(lib.fem_bsr_pattern_device(num_nodes, num_elements, npe, _dptr(conn_ptr), _dptr(conn), _dptr(rowp), None,
+                                      0, ctypes.byref(nb), _stream(stream)))
+    cols = torch.empty(max(nb.value, 1), dtype=torch.int32, device=conn.device)
+    _check(lib.fem_bsr_pattern_device(num_nodes, num_elements, npe, _dptr(conn_ptr), _dptr(conn), _dptr(rowp),
+                                      _dptr(cols), cols.numel(), ctypes.byref(nb), _stream(stream)))
+    return rowp, cols[: nb.value]
+
+
+def element_bcsr_map_device(conn_ptr, conn, num_elements, npe, num_rows, rowp, cols, bsize=2, stream=None):
+    """Device-built element->BSR map (torch int32 tensor of num_elements * npe^2 entries)."""
+    import torch
+
+    out = torch.empty(max(num_elements * npe * npe, 1), dtype=torch.int32, device=conn.device)
+    _check(load().fem_element_bcsr_map_device(_dptr(conn_ptr), _dptr(conn), num_elements, npe, num_rows, _dptr(rowp),
+                                              _dptr(cols), bsize, _dptr(out), _stream(stream)))
+    return out[: num_elements * npe * npe]
 
 
 def element_colouring(mesh: Mesh):

@@ -101,6 +101,20 @@ int fem_bsr_pattern_cols(int numNodes, int numElements, int nodesPerElement, con
  * (FEM_ERR_MAP_MISS), not a printed warning with the entry left unset (TACSHelpers.h:176-178). */
 int fem_element_bcsr_map(const int *connPtr, const int *conn, int numElements, int nodesPerElement, int numRows,
                          const int *rowp, const int *cols, int bsize, int *map);
+/* ---- BSR pattern and element map on the device ---------------------------------------------------
+ * The same pattern and map as fem_bsr_pattern_rowp/_cols and fem_element_bcsr_map (identical arrays), built
+ * on the GPU from device arrays, in work ordered on `stream` (a hipStream_t; NULL = default stream). These
+ * replace the host set-up the reference leaves to TACS (createSchurMat, verification/TestKernels.cpp:105)
+ * and generateElementBCSRMap (verification/TACSHelpers.h:138-182). Each call allocates and frees its own
+ * scratch and returns after its work has finished. connPtr entries must index valid conn entries, as for
+ * every entry point here.
+ * fem_bsr_pattern_device fills rowp (numNodes + 1 entries) and *numBlocks. Pass cols = NULL to size the
+ * pattern, then call again with cols of colsCapacity >= *numBlocks entries. FEM_ERR_UNSUPPORTED if a node's
+ * elements hold more than 1024 node references (the host builder has no such limit). */
+int fem_bsr_pattern_device(int numNodes, int numElements, int nodesPerElement, const int *connPtr, const int *conn,
+                           int *rowp, int *cols, int64_t colsCapacity, int64_t *numBlocks, void *stream);
+int fem_element_bcsr_map_device(const int *connPtr, const int *conn, int numElements, int nodesPerElement,
+                                int numRows, const int *rowp, const int *cols, int bsize, int *map, void *stream);
 /* Greedy element colouring: no two elements of one colour share a node. colour[e] in
  * 0..*numColours-1, assigned in element order. */
 int fem_element_colouring(int numNodes, int numElements, int nodesPerElement, const int *connPtr, const int *conn,
