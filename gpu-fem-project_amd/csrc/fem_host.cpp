@@ -719,6 +719,60 @@ int fem_element_bcsr_map(const int *connPtr, const int *conn, int numElements, i
   return FEM_OK;
 }
 
+// One value per line, "% .17g" (writeArrayToFile, verification/TACSHelpers.h:234-246).
+int fem_write_array(const double *values, int64_t n, const char *path) {
+  if (n < 0) return set_error(FEM_ERR_INVALID_ARGUMENT, "n = %lld is negative", (long long)n);
+  if (!path || (n > 0 && !values)) return set_error(FEM_ERR_NULL_POINTER, "values/path is NULL");
+  FILE *fp = std::fopen(path, "w");
+  if (!fp) return set_error(FEM_ERR_INVALID_ARGUMENT, "cannot open %s for writing", path);
+  for (int64_t i = 0; i < n; i++) std::fprintf(fp, "% .17g\n", values[i]);
+  const bool failed = std::ferror(fp) != 0;
+  if (std::fclose(fp) != 0 || failed) return set_error(FEM_ERR_INVALID_ARGUMENT, "writing %s failed", path);
+  return FEM_OK;
+}
+
+// The reference's matrix file (writeBCSRMatToFile, verification/TACSHelpers.h:248-307): a header
+// "rows cols nnz" in scalar entries, then one "row col value" line per entry of each block, 0-based, blocks
+// row-major, value "% .17g". maxRows / maxCols > 0 keep the leading block rows / block columns only (a row's
+// columns are sorted, so its blocks stop at the first column beyond the limit, as in the reference).
+int fem_write_bsr_matrix(int nrows, int ncols, int bsize, const int *rowp, const int *cols, const double *values,
+                         int maxRows, int maxCols, const char *path) {
+  if (nrows < 0 || ncols < 0 || bsize < 1) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "nrows=%d ncols=%d bsize=%d", nrows, ncols, bsize);
+  }
+  if (!path || !rowp || (rowp[nrows] > 0 && (!cols || !values))) {
+    return set_error(FEM_ERR_NULL_POINTER, "rowp/cols/values/path is NULL");
+  }
+  const int64_t bs2 = int64_t(bsize) * bsize;
+  int64_t nnz = int64_t(rowp[nrows]) * bs2;
+  const bool limited = maxRows > 0 || maxCols > 0;
+  if (maxRows > 0) nrows = std::min(maxRows, nrows);
+  if (maxCols > 0) ncols = std::min(maxCols, ncols);
+  if (limited) {
+    nnz = 0;
+    for (int i = 0; i < nrows; i++) {
+      for (int k = rowp[i]; k < rowp[i + 1] && cols[k] < ncols; k++) nnz += bs2;
+    }
+  }
+  FILE *fp = std::fopen(path, "w");
+  if (!fp) return set_error(FEM_ERR_INVALID_ARGUMENT, "cannot open %s for writing", path);
+  std::fprintf(fp, "%lld %lld %lld\n", (long long)nrows * bsize, (long long)ncols * bsize, (long long)nnz);
+  for (int i = 0; i < nrows; i++) {
+    for (int k = rowp[i]; k < rowp[i + 1] && cols[k] < ncols; k++) {
+      const double *block = values + int64_t(k) * bs2;
+      for (int r = 0; r < bsize; r++) {
+        for (int c = 0; c < bsize; c++) {
+          std::fprintf(fp, "%lld %lld % .17g\n", (long long)i * bsize + r, (long long)cols[k] * bsize + c,
+                       block[r * bsize + c]);
+        }
+      }
+    }
+  }
+  const bool failed = std::ferror(fp) != 0;
+  if (std::fclose(fp) != 0 || failed) return set_error(FEM_ERR_INVALID_ARGUMENT, "writing %s failed", path);
+  return FEM_OK;
+}
+
 int fem_element_colouring(int numNodes, int numElements, int nodesPerElement, const int *connPtr, const int *conn,
                           int *colour, int *numColours) {
   int rc = check_connectivity(numNodes, numElements, nodesPerElement, connPtr, conn);

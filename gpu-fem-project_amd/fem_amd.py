@@ -63,6 +63,8 @@ SIGNATURES = {
     "fem_bsr_pattern_cols": ([_I, _I, _I, _P, _P, _P, _P], _I),
     "fem_element_bcsr_map": ([_P, _P, _I, _I, _I, _P, _P, _I, _P], _I),
     "fem_bsr_pattern_device": ([_I, _I, _I, _P, _P, _P, _P, _I64, _PI64, _P], _I),
+    "fem_write_array": ([_P, _I64, ctypes.c_char_p], _I),
+    "fem_write_bsr_matrix": ([_I, _I, _I, _P, _P, _P, _I, _I, ctypes.c_char_p], _I),
     "fem_element_bcsr_map_device": ([_P, _P, _I, _I, _I, _P, _P, _I, _P, _P], _I),
     "fem_element_colouring": ([_I, _I, _I, _P, _P, _P, ctypes.POINTER(_I)], _I),
     "fem_residual": ([_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _I, _P, ctypes.POINTER(_D)], _I),
@@ -267,6 +269,21 @@ def element_bcsr_map(mesh: Mesh, pattern: Pattern, bsize: int = 2) -> np.ndarray
     _check(load().fem_element_bcsr_map(_ptr(cp), _ptr(mesh.conn), mesh.num_elements, mesh.npe, mesh.num_nodes,
                                        _ptr(pattern.rowp), _ptr(pattern.cols), bsize, _ptr(out)))
     return out
+
+
+def write_array(values: np.ndarray, path: str) -> None:
+    """The reference's vector file: one value per line, "% .17g"."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    _check(load().fem_write_array(_ptr(v), v.size, os.fsencode(path)))
+
+
+def write_bsr_matrix(pattern: "Pattern", values: np.ndarray, path: str, bsize: int = 2, ncols: Optional[int] = None,
+                     max_rows: int = -1, max_cols: int = -1) -> None:
+    """The reference's matrix file: "rows cols nnz", then 0-based "row col value" lines."""
+    nrows = pattern.rowp.size - 1
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    _check(load().fem_write_bsr_matrix(nrows, nrows if ncols is None else ncols, bsize, _ptr(pattern.rowp),
+                                       _ptr(pattern.cols), _ptr(v), max_rows, max_cols, os.fsencode(path)))
 
 
 def bsr_pattern_device(num_nodes, num_elements, npe, conn_ptr, conn, stream=None):

@@ -101,6 +101,16 @@ int fem_bsr_pattern_cols(int numNodes, int numElements, int nodesPerElement, con
  * (FEM_ERR_MAP_MISS), not a printed warning with the entry left unset (TACSHelpers.h:176-178). */
 int fem_element_bcsr_map(const int *connPtr, const int *conn, int numElements, int nodesPerElement, int numRows,
                          const int *rowp, const int *cols, int bsize, int *map);
+/* ---- Output files in the reference's formats (host arrays) -----------------------------------------
+ * fem_write_array: one value per line, "% .17g" (writeArrayToFile, verification/TACSHelpers.h:234-246).
+ * fem_write_bsr_matrix: header "rows cols nnz" in scalar entries, then "row col value" per entry, 0-based,
+ * blocks in row order and each block row-major, "% .17g" (writeBCSRMatToFile, TACSHelpers.h:248-307);
+ * maxRows / maxCols > 0 keep the leading block rows / columns, as the reference's optional limits do. A
+ * file that cannot be opened or written is an error status (the reference prints and continues). */
+int fem_write_array(const double *values, int64_t n, const char *path);
+int fem_write_bsr_matrix(int nrows, int ncols, int bsize, const int *rowp, const int *cols, const double *values,
+                         int maxRows, int maxCols, const char *path);
+
 /* ---- BSR pattern and element map on the device ---------------------------------------------------
  * The same pattern and map as fem_bsr_pattern_rowp/_cols and fem_element_bcsr_map (identical arrays), built
  * on the GPU from device arrays, in work ordered on `stream` (a hipStream_t; NULL = default stream). These
