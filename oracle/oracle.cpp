@@ -148,6 +148,53 @@ inline void weakRes(const double Ux[4], double E, double nu, double t, double sc
   out[2] = out[1];
 }
 
+// planeStressWeakRes, NONLINEAR strain (FEKernels.h:286-315 with GreenStrainType::NONLINEAR): the Green strain
+// E = (Ux + Ux^T + Ux^T Ux) / 2, S = 2 mu E + lambda tr(E) I, energy tr(E S) seeded with scale*t/2, so the
+// reverse sweep gives dW/dUx = scale*t * F S with F = I + Ux (dE = sym(F^T dUx), tr(dE S) = (F S) : dUx).
+inline void greenStress(const double Ux[4], double mu, double lambda, double S[4]) {
+  const double exx = Ux[0] + 0.5 * (Ux[0] * Ux[0] + Ux[2] * Ux[2]);
+  const double eyy = Ux[3] + 0.5 * (Ux[1] * Ux[1] + Ux[3] * Ux[3]);
+  const double exy = 0.5 * (Ux[1] + Ux[2] + Ux[0] * Ux[1] + Ux[2] * Ux[3]);
+  const double tr = exx + eyy;
+  S[0] = 2 * mu * exx + lambda * tr;
+  S[3] = 2 * mu * eyy + lambda * tr;
+  S[1] = S[2] = 2 * mu * exy;
+}
+
+inline void weakResNonlinear(const double Ux[4], double E, double nu, double t, double scale, double out[4]) {
+  const double mu = 0.5 * E / (1.0 + nu);
+  const double lambda = 2 * mu * nu / (1.0 - nu);
+  double S[4];
+  greenStress(Ux, mu, lambda, S);
+  const double F[4] = {1.0 + Ux[0], Ux[1], Ux[2], 1.0 + Ux[3]};
+  double FS[4];
+  matmul2(F, S, FS);
+  const double f = scale * t;
+  for (int i = 0; i < 4; i++) out[i] = f * FS[i];
+}
+
+// Forward (tangent) derivative of weakResNonlinear at Ux in direction Udot, as the reference's ADScalar
+// sweep computes it (FEKernels.h:440-497): scale*t * (Udot S + F Sdot), Sdot = C : sym(F^T Udot).
+inline void weakResNonlinearFwd(const double Ux[4], const double Udot[4], double E, double nu, double t,
+                                double scale, double out[4]) {
+  const double mu = 0.5 * E / (1.0 + nu);
+  const double lambda = 2 * mu * nu / (1.0 - nu);
+  double S[4];
+  greenStress(Ux, mu, lambda, S);
+  const double F[4] = {1.0 + Ux[0], Ux[1], Ux[2], 1.0 + Ux[3]};
+  // Edot = (F^T Udot + Udot^T F) / 2
+  const double exx = F[0] * Udot[0] + F[2] * Udot[2];
+  const double eyy = F[1] * Udot[1] + F[3] * Udot[3];
+  const double exy = 0.5 * (F[0] * Udot[1] + F[2] * Udot[3] + Udot[0] * F[1] + Udot[2] * F[3]);
+  const double tr = exx + eyy;
+  const double Sd[4] = {2 * mu * exx + lambda * tr, 2 * mu * exy, 2 * mu * exy, 2 * mu * eyy + lambda * tr};
+  double a[4], b[4];
+  matmul2(Udot, S, a);
+  matmul2(F, Sd, b);
+  const double f = scale * t;
+  for (int i = 0; i < 4; i++) out[i] = f * (a[i] + b[i]);
+}
+
 // addTransformStateGradSens, the branch selected for every order (FEKernels.h:167-196):
 // sens[a*2+i] += sum_k (dN_a . Jinv)_k * wr(i, k).
 template <int order>
@@ -197,8 +244,8 @@ void gather(const int *connPtr, const int *conn, int e, const double *states, co
   }
 }
 
-// Residual of one element (FEKernels.h:317-377 loop body).
-template <int order>
+// Residual of one element (FEKernels.h:317-377 loop body); NL selects the NONLINEAR strain.
+template <int order, bool NL = false>
 void elementResidual(const double *lu, const double *lx, double E, double nu, double t, double *lres) {
   constexpr int N = (order + 1) * (order + 1);
   for (int k = 0; k < 2 * N; k++) lres[k] = 0.0;
@@ -212,15 +259,17 @@ void elementResidual(const double *lu, const double *lx, double E, double nu, do
       const double detJ = det2(J);
       interpParamGradient<order>(xi, lu, dudxi);
       matmul2(dudxi, Ji, dudx);
-      weakRes(dudx, E, nu, t, w * detJ, wr);
+      if (NL) weakResNonlinear(dudx, E, nu, t, w * detJ, wr);
+      else weakRes(dudx, E, nu, t, w * detJ, wr);
       addTransformStateGradSens<order>(xi, wr, Ji, lres);
     }
   }
 }
 
 // Residual and Jacobian of one element (FEKernels.h:379-508 loop body). localMat is stored
-// transposed, localMat[col*2N + row], as the reference does (:492).
-template <int order>
+// transposed, localMat[col*2N + row], as the reference does (:492). NL selects the NONLINEAR strain, whose
+// tangent depends on the state.
+template <int order, bool NL = false>
 void elementJacobian(const double *lu, const double *lx, double E, double nu, double t, double *lres,
                      double *localMat) {
   constexpr int N = (order + 1) * (order + 1);
@@ -237,7 +286,8 @@ void elementJacobian(const double *lu, const double *lx, double E, double nu, do
       const double detJ = det2(J);
       interpParamGradient<order>(xi, lu, dudxi);
       matmul2(dudxi, Ji, dudx);
-      weakRes(dudx, E, nu, t, w * detJ, wr);
+      if (NL) weakResNonlinear(dudx, E, nu, t, w * detJ, wr);
+      else weakRes(dudx, E, nu, t, w * detJ, wr);
       addTransformStateGradSens<order>(xi, wr, Ji, lres);
       for (int ny = 0; ny <= order; ny++) {
         for (int nx = 0; nx <= order; nx++) {
@@ -251,7 +301,8 @@ void elementJacobian(const double *lu, const double *lx, double E, double nu, do
             Udot[s * 2] = dxDot[0];
             Udot[s * 2 + 1] = dxDot[1];
             double wrDot[4];
-            weakRes(Udot, E, nu, t, w * detJ, wrDot);
+            if (NL) weakResNonlinearFwd(dudx, Udot, E, nu, t, w * detJ, wrDot);
+            else weakRes(Udot, E, nu, t, w * detJ, wrDot);
             double col[D];
             for (int k = 0; k < D; k++) col[k] = 0.0;
             addTransformStateGradSens<order>(xi, wrDot, Ji, col);
@@ -328,7 +379,7 @@ int set_threads(int threads) {
 #endif
 }
 
-template <int order>
+template <int order, bool NL = false>
 int residualT(const int *connPtr, const int *conn, int first, int count, const double *states, const double *coords,
               double E, double nu, double t, double *residual, int threads) {
   constexpr int N = (order + 1) * (order + 1);
@@ -338,7 +389,7 @@ int residualT(const int *connPtr, const int *conn, int first, int count, const d
   for (int e = first; e < first + count; e++) {
     double lu[2 * N], lx[2 * N], lres[2 * N];
     gather<order>(connPtr, conn, e, states, coords, lu, lx);
-    elementResidual<order>(lu, lx, E, nu, t, lres);
+    elementResidual<order, NL>(lu, lx, E, nu, t, lres);
     for (int a = 0; a < N; a++) {
       const int n = conn[connPtr[e] + a];
       for (int s = 0; s < 2; s++) atomic_add(&residual[2 * n + s], lres[2 * a + s], atomic);
@@ -347,7 +398,7 @@ int residualT(const int *connPtr, const int *conn, int first, int count, const d
   return 0;
 }
 
-template <int order, bool gram>
+template <int order, bool gram, bool NL = false>
 int jacobianT(const int *connPtr, const int *conn, int first, int count, const double *states, const double *coords,
               double E, double nu, double t, const int *map, double *residual, double *values, int threads) {
   constexpr int N = (order + 1) * (order + 1);
@@ -362,7 +413,7 @@ int jacobianT(const int *connPtr, const int *conn, int first, int count, const d
     if (gram) {
       elementGram<order>(lu, lx, E, nu, t, lres, M.data());
     } else {
-      elementJacobian<order>(lu, lx, E, nu, t, lres, M.data());
+      elementJacobian<order, NL>(lu, lx, E, nu, t, lres, M.data());
     }
     // scatterMat (FEKernels.h:63-104): block (i, j) entry (r, c) += localMat[(2i+r)*2N + 2j+c].
     const int *m = map + int64_t(e) * N * N;
@@ -384,14 +435,14 @@ int jacobianT(const int *connPtr, const int *conn, int first, int count, const d
   return 0;
 }
 
-template <int order>
+template <int order, bool NL = false>
 int elementT(const double *lx, const double *lu, double E, double nu, double t, int gram, double *lres, double *K) {
   constexpr int D = 2 * (order + 1) * (order + 1);
-  if (gram) {
+  if (gram && !NL) {
     elementGram<order>(lu, lx, E, nu, t, lres, K);
   } else {
     std::vector<double> M(size_t(D) * D);
-    elementJacobian<order>(lu, lx, E, nu, t, lres, M.data());
+    elementJacobian<order, NL>(lu, lx, E, nu, t, lres, M.data());
     // Return row-major K: K[r*D + c] = localMat[c*D + r].
     for (int r = 0; r < D; r++) {
       for (int c = 0; c < D; c++) K[r * D + c] = M[c * D + r];
@@ -465,6 +516,46 @@ int oracle_jacobian(int order, const int *connPtr, const int *conn, int first, i
     default: return -1;
   }
 #undef ORACLE_JAC
+}
+
+// NONLINEAR-strain versions of oracle_element / oracle_residual / oracle_jacobian (the reference's kernels
+// with GreenStrainType::NONLINEAR; its dispatchers instantiate LINEAR only).
+int oracle_element_nl(int order, const double *lx, const double *lu, double E, double nu, double t, double *lres,
+                      double *K) {
+  switch (order) {
+    case 1: return elementT<1, true>(lx, lu, E, nu, t, 0, lres, K);
+    case 2: return elementT<2, true>(lx, lu, E, nu, t, 0, lres, K);
+    case 3: return elementT<3, true>(lx, lu, E, nu, t, 0, lres, K);
+    case 4: return elementT<4, true>(lx, lu, E, nu, t, 0, lres, K);
+    default: return -1;
+  }
+}
+
+int oracle_residual_nl(int order, const int *connPtr, const int *conn, int first, int count, const double *states,
+                       const double *coords, double E, double nu, double t, double *residual, int threads) {
+  switch (order) {
+    case 1: return residualT<1, true>(connPtr, conn, first, count, states, coords, E, nu, t, residual, threads);
+    case 2: return residualT<2, true>(connPtr, conn, first, count, states, coords, E, nu, t, residual, threads);
+    case 3: return residualT<3, true>(connPtr, conn, first, count, states, coords, E, nu, t, residual, threads);
+    case 4: return residualT<4, true>(connPtr, conn, first, count, states, coords, E, nu, t, residual, threads);
+    default: return -1;
+  }
+}
+
+int oracle_jacobian_nl(int order, const int *connPtr, const int *conn, int first, int count, const double *states,
+                       const double *coords, double E, double nu, double t, const int *map, double *residual,
+                       double *values, int threads) {
+#define ORACLE_JAC_NL(P)                                                                                     \
+  return jacobianT<P, false, true>(connPtr, conn, first, count, states, coords, E, nu, t, map, residual,    \
+                                   values, threads)
+  switch (order) {
+    case 1: ORACLE_JAC_NL(1);
+    case 2: ORACLE_JAC_NL(2);
+    case 3: ORACLE_JAC_NL(3);
+    case 4: ORACLE_JAC_NL(4);
+    default: return -1;
+  }
+#undef ORACLE_JAC_NL
 }
 
 // Brute-force block pattern: the set of (conn[e,i], conn[e,j]) pairs, rows sorted. Two calls: with

@@ -27,6 +27,9 @@ _SIGS = {
     "oracle_element_bcsr_map": ([_P, _P, _I, _I, _P, _P, _I, _P], _I),
     "oracle_time_res_jac": ([_I, _P, _P, _I, _I, _P, _P, _D, _D, _D, _P, _P, _P, _I], _D),
     "oracle_max_threads": ([], _I),
+    "oracle_element_nl": ([_I, _P, _P, _D, _D, _D, _P, _P], _I),
+    "oracle_residual_nl": ([_I, _P, _P, _I, _I, _P, _P, _D, _D, _D, _P, _I], _I),
+    "oracle_jacobian_nl": ([_I, _P, _P, _I, _I, _P, _P, _D, _D, _D, _P, _P, _P, _I], _I),
 }
 _libs: dict = {}
 
@@ -64,30 +67,41 @@ def lagrange1d(order: int, x: float, k: int):
     return n.value, d.value
 
 
-def element(order: int, lx: np.ndarray, lu: np.ndarray, E: float, nu: float, t: float, gram: bool = False):
-    """Residual (2N) and row-major stiffness (2N x 2N) of one element from nodal coords/states."""
+def element(order: int, lx: np.ndarray, lu: np.ndarray, E: float, nu: float, t: float, gram: bool = False,
+            nonlinear: bool = False):
+    """Residual (2N) and row-major stiffness (2N x 2N) of one element from nodal coords/states (nonlinear:
+    the NONLINEAR Green strain, whose stiffness is the tangent at lu)."""
     N = (order + 1) ** 2
     lx = np.ascontiguousarray(lx, dtype=np.float64).reshape(-1)
     lu = np.ascontiguousarray(lu, dtype=np.float64).reshape(-1)
     res = np.zeros(2 * N)
     K = np.zeros((2 * N, 2 * N))
-    assert load().oracle_element(order, _p(lx), _p(lu), E, nu, t, int(gram), _p(res), _p(K)) == 0
+    if nonlinear:
+        assert load().oracle_element_nl(order, _p(lx), _p(lu), E, nu, t, _p(res), _p(K)) == 0
+    else:
+        assert load().oracle_element(order, _p(lx), _p(lu), E, nu, t, int(gram), _p(res), _p(K)) == 0
     return res, K
 
 
-def residual(mesh, states, E, nu, t, threads: int = 1, first: int = 0, count=None):
+def residual(mesh, states, E, nu, t, threads: int = 1, first: int = 0, count=None, nonlinear: bool = False):
     count = mesh.num_elements - first if count is None else count
     out = np.zeros(mesh.num_nodes * 2)
     cp = mesh.conn_ptr
-    assert load().oracle_residual(mesh.order, _p(cp), _p(mesh.conn), first, count, _p(states), _p(mesh.coords), E,
-                                  nu, t, _p(out), threads) == 0
+    fn = load().oracle_residual_nl if nonlinear else load().oracle_residual
+    assert fn(mesh.order, _p(cp), _p(mesh.conn), first, count, _p(states), _p(mesh.coords), E, nu, t, _p(out),
+              threads) == 0
     return out
 
 
-def jacobian(mesh, states, E, nu, t, bcsr_map, num_blocks, with_residual=True, gram=False, threads: int = 1):
+def jacobian(mesh, states, E, nu, t, bcsr_map, num_blocks, with_residual=True, gram=False, threads: int = 1,
+             nonlinear: bool = False):
     values = np.zeros(num_blocks * 4)
     res = np.zeros(mesh.num_nodes * 2) if with_residual else None
     cp = mesh.conn_ptr
+    if nonlinear:
+        assert load().oracle_jacobian_nl(mesh.order, _p(cp), _p(mesh.conn), 0, mesh.num_elements, _p(states),
+                                         _p(mesh.coords), E, nu, t, _p(bcsr_map), _p(res), _p(values), threads) == 0
+        return res, values
     assert load().oracle_jacobian(mesh.order, _p(cp), _p(mesh.conn), 0, mesh.num_elements, _p(states),
                                   _p(mesh.coords), E, nu, t, _p(bcsr_map), _p(res), _p(values), int(gram),
                                   threads) == 0
