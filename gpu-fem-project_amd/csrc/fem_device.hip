@@ -463,6 +463,207 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   }
 }
 
+// ---- NONLINEAR (Green) strain: element-parallel drop-in paths ------------------------------------------
+// The reference's kernels with GreenStrainType::NONLINEAR (element/FEKernels.h:286-315, 585-1006): Green
+// strain E = (Ux + Ux^T + Ux^T Ux) / 2, S = 2 mu E + lambda tr(E) I, weak residual s F S with F = I + Ux
+// (the reverse sweep of the energy tr(E S) s / 2), and its tangent
+//   K_ik(a, b) = sum_q s [mu (F F^T)_ik (n_a . n_b) + mu (F n_b)_i (F n_a)_k + lambda (F n_a)_i (F n_b)_k
+//                         + delta_ik n_a^T S n_b],
+// n_a the physical gradient of node a. FP64 only.
+template <int P>
+struct __align__(16) WaveLDSNL {
+  using O = Ord<P>;
+  static constexpr int NG = 20;  // doubles per quadrature point (see stage_geometry_nl)
+  static constexpr int GEO_STRIDE = 2 * odd_units(O::Q * NG / 2, 16);
+  union {
+    double nodal[O::EPW][O::NODAL_STRIDE];  // x, y, u, v of each element node
+    double2 pub[O::EPW][O::N][2];           // n_b and F n_b of the current quadrature point
+  };
+  double geo[O::EPW][GEO_STRIDE];
+};
+
+// Lane (el, q): geometry, deformation and stresses at quadrature point q, stored as
+// [J^-1 (4), s, s F S (4), F (4), F F^T (xx, xy, yy), s S (xx, xy, yy), 0].
+template <int P>
+__device__ __forceinline__ void stage_geometry_nl(WaveLDSNL<P> &W, const Basis1D &B, const Material &m, int el,
+                                                  int q) {
+  constexpr int NP = Ord<P>::NP;
+  const int qx = q % NP, qy = q / NP;
+  double J00 = 0, J01 = 0, J10 = 0, J11 = 0, U00 = 0, U01 = 0, U10 = 0, U11 = 0;
+#pragma unroll
+  for (int ay = 0; ay < NP; ay++) {
+#pragma unroll
+    for (int ax = 0; ax < NP; ax++) {
+      const int a = ay * NP + ax;
+      const double gx = B.phi[ay][qy] * B.dphi[ax][qx];
+      const double gy = B.phi[ax][qx] * B.dphi[ay][qy];
+      const double2 xy = *reinterpret_cast<const double2 *>(&W.nodal[el][4 * a]);
+      const double2 uv = *reinterpret_cast<const double2 *>(&W.nodal[el][4 * a + 2]);
+      J00 = fma(xy.x, gx, J00);
+      J01 = fma(xy.x, gy, J01);
+      J10 = fma(xy.y, gx, J10);
+      J11 = fma(xy.y, gy, J11);
+      U00 = fma(uv.x, gx, U00);
+      U01 = fma(uv.x, gy, U01);
+      U10 = fma(uv.y, gx, U10);
+      U11 = fma(uv.y, gy, U11);
+    }
+  }
+  const double detJ = J00 * J11 - J01 * J10;
+  const double inv = 1.0 / detJ;
+  const double i00 = J11 * inv, i01 = -J01 * inv, i10 = -J10 * inv, i11 = J00 * inv;
+  // displacement gradient Ux = dU/dxi J^-1
+  const double G00 = U00 * i00 + U01 * i10, G01 = U00 * i01 + U01 * i11;
+  const double G10 = U10 * i00 + U11 * i10, G11 = U10 * i01 + U11 * i11;
+  const double exx = G00 + 0.5 * (G00 * G00 + G10 * G10);
+  const double eyy = G11 + 0.5 * (G01 * G01 + G11 * G11);
+  const double exy = 0.5 * (G01 + G10 + G00 * G01 + G10 * G11);
+  const double tr = exx + eyy;
+  const double Sxx = 2 * m.mu * exx + m.lambda * tr, Syy = 2 * m.mu * eyy + m.lambda * tr, Sxy = 2 * m.mu * exy;
+  const double F00 = 1 + G00, F01 = G01, F10 = G10, F11 = 1 + G11;
+  const double s = (B.w[qx] * B.w[qy]) * detJ * m.t;
+  double *g = &W.geo[el][WaveLDSNL<P>::NG * q];
+  g[0] = i00;
+  g[1] = i01;
+  g[2] = i10;
+  g[3] = i11;
+  g[4] = s;
+  g[5] = s * (F00 * Sxx + F01 * Sxy);
+  g[6] = s * (F00 * Sxy + F01 * Syy);
+  g[7] = s * (F10 * Sxx + F11 * Sxy);
+  g[8] = s * (F10 * Sxy + F11 * Syy);
+  g[9] = F00;
+  g[10] = F01;
+  g[11] = F10;
+  g[12] = F11;
+  g[13] = F00 * F00 + F01 * F01;
+  g[14] = F00 * F10 + F01 * F11;
+  g[15] = F10 * F10 + F11 * F11;
+  g[16] = s * Sxx;
+  g[17] = s * Sxy;
+  g[18] = s * Syy;
+  g[19] = 0;
+}
+
+// Lane (el, b): residual of node b and (JAC) the blocks K(a, b) of every row a, in chunks of CH rows; the
+// gradients of the chunk's rows come through the per-wave LDS one quadrature point at a time. All lanes of
+// the wavefront must call this; `valid` masks the lane's writes and emits. where/emit as for
+// stage_stiffness.
+template <int P, bool JAC, typename Where, typename Emit>
+__device__ __forceinline__ void stage_stiffness_nl(WaveLDSNL<P> &W, const double2 *gref, const Material &m, int el,
+                                                   int b, bool valid, double &Rx, double &Ry, Where &&where,
+                                                   Emit &&emit) {
+  using O = Ord<P>;
+  constexpr int N = O::N, Q = O::Q, NG = WaveLDSNL<P>::NG, CH = 4;
+  const int elc = el < O::EPW ? el : O::EPW - 1;
+  const int bc = b < N ? b : 0;
+  const double mu = m.mu, lam = m.lambda;
+  Rx = 0;
+  Ry = 0;
+#pragma unroll 1
+  for (int c0 = 0; c0 < (JAC ? N : 1); c0 += CH) {
+    double A[CH][3], Tm[CH][4], H[CH];
+#pragma unroll
+    for (int j = 0; j < CH; j++) {
+      A[j][0] = A[j][1] = A[j][2] = 0;
+      Tm[j][0] = Tm[j][1] = Tm[j][2] = Tm[j][3] = 0;
+      H[j] = 0;
+    }
+#pragma unroll 1
+    for (int q = 0; q < Q; q++) {
+      const double *g = &W.geo[elc][NG * q];
+      const double2 gb = gref[q * N + bc];
+      const double nx = gb.x * g[0] + gb.y * g[2];
+      const double ny = gb.x * g[1] + gb.y * g[3];
+      if (c0 == 0) {
+        Rx += g[5] * nx + g[6] * ny;
+        Ry += g[7] * nx + g[8] * ny;
+      }
+      if (JAC) {
+        const double s = g[4];
+        const double px = g[9] * nx + g[10] * ny, py = g[11] * nx + g[12] * ny;
+        const double spx = s * px, spy = s * py;
+        const double qx = g[16] * nx + g[17] * ny, qy = g[17] * nx + g[18] * ny;
+        if (valid) {
+          W.pub[el][b][0] = make_double2(nx, ny);
+          W.pub[el][b][1] = make_double2(px, py);
+        }
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+          const int a = c0 + j < N ? c0 + j : N - 1;
+          const double2 na = W.pub[elc][a][0], pa = W.pub[elc][a][1];
+          const double sd = s * (na.x * nx + na.y * ny);
+          A[j][0] = fma(sd, g[13], A[j][0]);
+          A[j][1] = fma(sd, g[14], A[j][1]);
+          A[j][2] = fma(sd, g[15], A[j][2]);
+          Tm[j][0] = fma(pa.x, spx, Tm[j][0]);
+          Tm[j][1] = fma(pa.x, spy, Tm[j][1]);
+          Tm[j][2] = fma(pa.y, spx, Tm[j][2]);
+          Tm[j][3] = fma(pa.y, spy, Tm[j][3]);
+          H[j] = fma(na.x, qx, fma(na.y, qy, H[j]));
+        }
+        wave_sync();  // the next point republishes into the same slots
+      }
+    }
+    if (JAC && valid) {
+      int at[CH];
+#pragma unroll
+      for (int j = 0; j < CH; j++) at[j] = c0 + j < N ? where(c0 + j, b) : -1;
+#pragma unroll
+      for (int j = 0; j < CH; j++) {
+        if (c0 + j >= N) break;
+        // K_ik = mu A_ik + mu T_ki + lambda T_ik + delta_ik H, T_ik = sum_q s (F n_a)_i (F n_b)_k
+        const double k[4] = {mu * A[j][0] + (mu + lam) * Tm[j][0] + H[j], mu * A[j][1] + mu * Tm[j][2] + lam * Tm[j][1],
+                             mu * A[j][1] + mu * Tm[j][1] + lam * Tm[j][2], mu * A[j][2] + (mu + lam) * Tm[j][3] + H[j]};
+        emit(at[j], k);
+      }
+    }
+  }
+}
+
+template <int P, int WAVES, bool RES, bool JAC>
+__global__ __launch_bounds__(WAVES * 64) void element_kernel_nl(const ElemArgs args) {
+  using O = Ord<P>;
+  constexpr int N = O::N;
+  __shared__ WaveLDSNL<P> wl[WAVES];
+  __shared__ Basis1D basis;
+  __shared__ double2 gref[O::Q * O::N];
+  copy_basis(basis, args.basis);
+  fill_gref<P>(gref, args.basis);
+  __syncthreads();
+
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int el = lane / N, idx = lane % N;
+  WaveLDSNL<P> &W = wl[wave];
+  const int64_t slot = (int64_t(blockIdx.x) * WAVES + wave) * O::EPW + el;
+  const bool valid = lane < O::LANES && slot < args.count;
+  int eid = 0, node = 0;
+  if (valid) {
+    eid = args.first + int(slot);
+    node = args.conn[args.connPtr[eid] + idx];
+    store_nodal(W, el, idx, *reinterpret_cast<const double2 *>(args.coords + 2 * int64_t(node)),
+                *reinterpret_cast<const double2 *>(args.states + 2 * int64_t(node)));
+  }
+  wave_sync();
+  if (valid) stage_geometry_nl<P>(W, basis, args.mat, el, idx);
+  wave_sync();
+  const int *mel = args.map + int64_t(eid) * N * N;
+  double Rx, Ry;
+  auto where = [&](int row, int col) __attribute__((always_inline)) { return mel[row * N + col]; };
+  stage_stiffness_nl<P, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, where,
+                             [&](int at, const double(&k)[4]) __attribute__((always_inline)) {
+                               double *blk = args.values + at;
+#pragma unroll
+                               for (int c = 0; c < 4; c++) unsafeAtomicAdd(blk + c, k[c]);
+                             });
+  if (RES && valid) {
+    double *r = args.residual + 2 * int64_t(node);
+    unsafeAtomicAdd(r, Rx);
+    unsafeAtomicAdd(r + 1, Ry);
+  }
+}
+
 // ---- owner-computes kernels (resident plans, write-once) -------------------------------------------
 // One assembly of one resident plan: the plan's device arrays and the request's inputs/outputs.
 struct MeshJob {
@@ -1650,6 +1851,24 @@ void launch_element_order(int order, const ElemArgs &a, int precision, hipStream
   }
 }
 
+template <int P, bool RES, bool JAC>
+void launch_element_nl(const ElemArgs &a, hipStream_t stream) {
+  constexpr int W = elem_waves(P);
+  constexpr int per = W * Ord<P>::EPW;
+  const int grid = (a.count + per - 1) / per;
+  if (grid > 0) element_kernel_nl<P, W, RES, JAC><<<grid, W * 64, 0, stream>>>(a);
+}
+
+template <bool RES, bool JAC>
+void launch_element_nl_order(int order, const ElemArgs &a, hipStream_t stream) {
+  switch (order) {
+    case 1: launch_element_nl<1, RES, JAC>(a, stream); break;
+    case 2: launch_element_nl<2, RES, JAC>(a, stream); break;
+    case 3: launch_element_nl<3, RES, JAC>(a, stream); break;
+    default: launch_element_nl<4, RES, JAC>(a, stream); break;
+  }
+}
+
 // One launch group: jobs of one order with the same outputs.
 struct OwnerLaunch {
   OwnerArgs args;
@@ -1848,6 +2067,55 @@ int fem_jacobian(int order, const int *connPtr, const int *conn, int numElements
   return timed(s, elapsedSeconds, [&]() {
     if (residual) launch_element_order<true, true, true>(order, a, precision, s);
     else launch_element_order<false, true, true>(order, a, precision, s);
+    return FEM_OK;
+  });
+}
+
+int fem_residual_nonlinear(int order, const int *connPtr, const int *conn, int numElements, const double *states,
+                           const double *nodeCoords, double E, double nu, double t, double *residual, int precision,
+                           void *stream, double *elapsedSeconds) {
+  int rc = check_common(order, connPtr, conn, numElements, states, nodeCoords, precision);
+  if (rc) return rc;
+  if (precision != FEM_PREC_FP64) return set_error(FEM_ERR_UNSUPPORTED, "the NONLINEAR strain runs in FP64 only");
+  if (numElements > 0 && !residual) return set_error(FEM_ERR_NULL_POINTER, "residual is NULL");
+  ElemArgs a{};
+  a.connPtr = connPtr;
+  a.conn = conn;
+  a.count = numElements;
+  a.states = states;
+  a.coords = nodeCoords;
+  a.residual = residual;
+  if ((rc = make_basis(order, a.basis)) || (rc = make_material(E, nu, t, a.mat))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return timed(s, elapsedSeconds, [&]() {
+    launch_element_nl_order<true, false>(order, a, s);
+    return FEM_OK;
+  });
+}
+
+int fem_jacobian_nonlinear(int order, const int *connPtr, const int *conn, int numElements, const double *states,
+                           const double *nodeCoords, double E, double nu, double t, const int *elementBCSRMap,
+                           double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds) {
+  int rc = check_common(order, connPtr, conn, numElements, states, nodeCoords, precision);
+  if (rc) return rc;
+  if (precision != FEM_PREC_FP64) return set_error(FEM_ERR_UNSUPPORTED, "the NONLINEAR strain runs in FP64 only");
+  if (numElements > 0 && (!elementBCSRMap || !matEntries)) {
+    return set_error(FEM_ERR_NULL_POINTER, "elementBCSRMap/matEntries is NULL");
+  }
+  ElemArgs a{};
+  a.connPtr = connPtr;
+  a.conn = conn;
+  a.count = numElements;
+  a.states = states;
+  a.coords = nodeCoords;
+  a.map = elementBCSRMap;
+  a.residual = residual;
+  a.values = matEntries;
+  if ((rc = make_basis(order, a.basis)) || (rc = make_material(E, nu, t, a.mat))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return timed(s, elapsedSeconds, [&]() {
+    if (residual) launch_element_nl_order<true, true>(order, a, s);
+    else launch_element_nl_order<false, true>(order, a, s);
     return FEM_OK;
   });
 }

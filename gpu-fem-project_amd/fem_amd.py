@@ -69,6 +69,8 @@ SIGNATURES = {
     "fem_element_colouring": ([_I, _I, _I, _P, _P, _P, ctypes.POINTER(_I)], _I),
     "fem_residual": ([_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _I, _P, ctypes.POINTER(_D)], _I),
     "fem_jacobian": ([_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _P, _P, _I, _P, ctypes.POINTER(_D)], _I),
+    "fem_residual_nonlinear": ([_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _I, _P, ctypes.POINTER(_D)], _I),
+    "fem_jacobian_nonlinear": ([_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _P, _P, _I, _P, ctypes.POINTER(_D)], _I),
     "fem_jacobian_coloured": (
         [_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _P, _P, _I, _P, _P, _I, _P, ctypes.POINTER(_D)],
         _I,
@@ -336,22 +338,23 @@ def _elapsed_arg(timed: bool):
 
 
 def residual(order, conn_ptr, conn, num_elements, states, coords, E, nu, t, out, precision=FEM_PREC_FP64,
-             stream=None, timed=False):
-    """out += R (device tensors). Returns seconds when timed."""
+             stream=None, timed=False, nonlinear=False):
+    """out += R (device tensors). Returns seconds when timed. nonlinear: the NONLINEAR (Green) strain."""
     sec = ctypes.c_double()
-    _check(load().fem_residual(order, _dptr(conn_ptr), _dptr(conn), num_elements, _dptr(states), _dptr(coords),
-                               E, nu, t, _dptr(out), precision, _stream(stream),
-                               ctypes.byref(sec) if timed else None))
+    fn = load().fem_residual_nonlinear if nonlinear else load().fem_residual
+    _check(fn(order, _dptr(conn_ptr), _dptr(conn), num_elements, _dptr(states), _dptr(coords), E, nu, t, _dptr(out),
+              precision, _stream(stream), ctypes.byref(sec) if timed else None))
     return sec.value if timed else None
 
 
 def jacobian(order, conn_ptr, conn, num_elements, states, coords, E, nu, t, bcsr_map, res, values,
-             precision=FEM_PREC_FP64, stream=None, timed=False):
-    """values += K through the map; res (optional) += R."""
+             precision=FEM_PREC_FP64, stream=None, timed=False, nonlinear=False):
+    """values += K through the map; res (optional) += R. nonlinear: the NONLINEAR (Green) strain's tangent."""
     sec = ctypes.c_double()
-    _check(load().fem_jacobian(order, _dptr(conn_ptr), _dptr(conn), num_elements, _dptr(states), _dptr(coords), E,
-                               nu, t, _dptr(bcsr_map), _dptr(res), _dptr(values), precision, _stream(stream),
-                               ctypes.byref(sec) if timed else None))
+    fn = load().fem_jacobian_nonlinear if nonlinear else load().fem_jacobian
+    _check(fn(order, _dptr(conn_ptr), _dptr(conn), num_elements, _dptr(states), _dptr(coords), E, nu, t,
+              _dptr(bcsr_map), _dptr(res), _dptr(values), precision, _stream(stream),
+              ctypes.byref(sec) if timed else None))
     return sec.value if timed else None
 
 

@@ -143,6 +143,16 @@ int fem_residual(int order, const int *connPtr, const int *conn, int numElements
 int fem_jacobian(int order, const int *connPtr, const int *conn, int numElements, const double *states,
                  const double *nodeCoords, double E, double nu, double t, const int *elementBCSRMap,
                  double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds);
+/* NONLINEAR (Green) strain versions of fem_residual / fem_jacobian (same arguments and accumulate semantics):
+ * the reference's kernels instantiated with GreenStrainType::NONLINEAR (element/FEKernels.h:286-315,
+ * 585-1006; its dispatchers runResidualKernel / runJacobianKernel, :1030-1136, fix LINEAR). The stiffness is
+ * the tangent at `states`. precision must be FEM_PREC_FP64 (FEM_ERR_UNSUPPORTED otherwise). */
+int fem_residual_nonlinear(int order, const int *connPtr, const int *conn, int numElements, const double *states,
+                           const double *nodeCoords, double E, double nu, double t, double *residual, int precision,
+                           void *stream, double *elapsedSeconds);
+int fem_jacobian_nonlinear(int order, const int *connPtr, const int *conn, int numElements, const double *states,
+                           const double *nodeCoords, double E, double nu, double t, const int *elementBCSRMap,
+                           double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds);
 /* Deterministic element-colouring variant of fem_jacobian (no reference counterpart beyond the
  * element-colouring requirement of the same path): elements are processed colour by colour from
  * colourElements (device, element ids grouped by colour, colourPtr (host) gives numColours+1
