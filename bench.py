@@ -489,9 +489,13 @@ def main():
     ap.add_argument("--side-requests", type=int, default=2000, help="serve-poisson: requests at 0.5 and 0.95 load")
     ap.add_argument("--pool", type=int, default=0, help="override the mesh-pool size (0: the workload's own)")
     ap.add_argument("--autotune", action="store_true", help="plan path: build the plan with FEM_PLAN_AUTOTUNE")
+    ap.add_argument("--strain", default="linear", choices=["linear", "nonlinear"],
+                    help="nonlinear: the Green-strain tangent (drop-in atomic path, FP64)")
     args = ap.parse_args()
     if args.autotune and args.path != "plan":
         ap.error("--autotune applies to --path plan only")
+    if args.strain == "nonlinear" and (args.path != "atomic" or args.precision == "mixed"):
+        ap.error("--strain nonlinear runs on --path atomic in FP64")
     if args.precision is None:
         args.precision = "mixed" if args.workload == "batch256-mixed" else "fp64"
     if args.gpus != 1:
@@ -540,7 +544,7 @@ def main():
             res.zero_()
             vals.zero_()
             F.jacobian(mesh.order, conn_ptr, conn, mesh.num_elements, states, coords, E_MOD, NU, THICK, dmap, res,
-                       vals, precision=prec, stream=stream)
+                       vals, precision=prec, stream=stream, nonlinear=args.strain == "nonlinear")
 
     for _ in range(args.warmup):
         step()
@@ -569,10 +573,21 @@ def main():
     if not args.no_check:
         # Size-independent properties on the full workload: R = K u, and the plan agrees with the
         # element-parallel atomic path (a different kernel and summation order).
-        y = torch.zeros_like(res)
-        F.bsr_spmv(d(pat.rowp), d(pat.cols), vals, states, y)
-        r_ku = float((y - res).abs().max() / res.abs().max())
-        check = {"R_vs_Ku": r_ku}
+        rowp_d, cols_d = d(pat.rowp), d(pat.cols)
+        if args.strain == "nonlinear":
+            # The tangent is the Hessian of the strain energy: symmetric, x^T K z = z^T K x.
+            x = torch.rand_like(res) - 0.5
+            z = torch.rand_like(res) - 0.5
+            kx, kz = torch.zeros_like(res), torch.zeros_like(res)
+            F.bsr_spmv(rowp_d, cols_d, vals, x, kx)
+            F.bsr_spmv(rowp_d, cols_d, vals, z, kz)
+            sym = float((z @ kx - x @ kz).abs() / (x.norm() * kz.norm()))
+            check = {"K_symmetry": sym}
+        else:
+            y = torch.zeros_like(res)
+            F.bsr_spmv(rowp_d, cols_d, vals, states, y)
+            r_ku = float((y - res).abs().max() / res.abs().max())
+            check = {"R_vs_Ku": r_ku}
         if plan is not None:
             res2 = torch.zeros_like(res)
             vals2 = torch.zeros_like(vals)
@@ -607,11 +622,11 @@ def main():
         "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]['desc']}",
                    "elements": mesh.num_elements, "nodes": mesh.num_nodes, "bsr_blocks": pat.num_blocks,
                    "path": args.path + ("+autotune" if args.autotune else ""), "precision": args.precision,
-                   "parallelism": "single GPU"},
+                   "strain": args.strain, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": ("owner_kernel" if args.path == "plan-det" else "owner_stream_kernel") if plan is not None
-                     else "element_kernel",
+                     else ("element_kernel_nl" if args.strain == "nonlinear" else "element_kernel"),
                      "algorithmic_bytes": b_alg, "kernel_ms": kernel_s * 1e3},
         "cpu_baseline": cpu,
         "checks": check,

@@ -397,6 +397,11 @@ __device__ __forceinline__ void stage_stiffness(WaveLDS<P, T> &W, const double2 
 }
 
 // ---- element-parallel kernel (drop-in atomic path and colour path) -----------------------------
+template <int P, typename T, bool JAC, typename Where, typename Emit>
+__device__ __forceinline__ void stage_stiffness_rows(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el,
+                                                     int b, bool valid, int count, int maxCount, double &Rx,
+                                                     double &Ry, Where &&where, Emit &&emit);
+
 struct ElemArgs {
   const int *connPtr;
   const int *conn;
@@ -441,7 +446,7 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   const int *mel = args.map + int64_t(eid) * N * N;
   double Rx, Ry;
   auto where = [&](int row, int col) __attribute__((always_inline)) { return mel[row * N + col]; };
-  stage_stiffness<P, T, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, where, [&](int at, const double(&k)[4]) __attribute__((always_inline)) {
+  auto emit = [&](int at, const double(&k)[4]) __attribute__((always_inline)) {
     double *blk = args.values + at;
     if (ATOMIC) {
 #pragma unroll
@@ -450,7 +455,16 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
 #pragma unroll
       for (int c = 0; c < 4; c++) blk[c] += k[c];
     }
-  });
+  };
+  if constexpr (JAC) {
+    // Row-major emission: every row of K_e in chunks, so one add instruction targets the same BSR row for
+    // all lanes of an element (neighbouring blocks) instead of a different row per lane.
+    if (valid) W.orow[el][idx] = (unsigned char)idx;
+    wave_sync();
+    stage_stiffness_rows<P, T, JAC>(W, gref, args.mat, el, idx, valid, N, N, Rx, Ry, where, emit);
+  } else {
+    stage_stiffness<P, T, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, where, emit);
+  }
   if (RES && valid) {
     double *r = args.residual + 2 * int64_t(node);
     if (ATOMIC) {
