@@ -397,7 +397,7 @@ __device__ __forceinline__ void stage_stiffness(WaveLDS<P, T> &W, const double2 
 }
 
 // ---- element-parallel kernel (drop-in atomic path and colour path) -----------------------------
-template <int P, typename T, bool JAC, typename Where, typename Emit>
+template <int P, typename T, bool JAC, bool ALL_LANES, typename Where, typename Emit>
 __device__ __forceinline__ void stage_stiffness_rows(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el,
                                                      int b, bool valid, int count, int maxCount, double &Rx,
                                                      double &Ry, Where &&where, Emit &&emit);
@@ -424,6 +424,8 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   __shared__ WaveLDS<P, T> wl[WAVES];
   __shared__ Basis1D basis;
   __shared__ double2 gref[O::Q * O::N];
+  __shared__ double stK[JAC ? WAVES : 1][4][64];  // Jacobian emission: each lane's block, by component
+  __shared__ int stAt[JAC ? WAVES : 1][64];        // and its destination (negative: none)
   copy_basis(basis, args.basis);
   fill_gref<P>(gref, args.basis);
   __syncthreads();
@@ -457,11 +459,30 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
     }
   };
   if constexpr (JAC) {
-    // Row-major emission: every row of K_e in chunks, so one add instruction targets the same BSR row for
-    // all lanes of an element (neighbouring blocks) instead of a different row per lane.
+    // Row-major emission: every row of K_e in chunks, so the lanes of an element hold neighbouring blocks of
+    // one BSR row. The blocks then pass through LDS so that each add instruction covers the four
+    // components of 16 blocks (lanes 4i..4i+3 take block i), 32 contiguous bytes per block.
     if (valid) W.orow[el][idx] = (unsigned char)idx;
     wave_sync();
-    stage_stiffness_rows<P, T, JAC>(W, gref, args.mat, el, idx, valid, N, N, Rx, Ry, where, emit);
+    auto emitRows = [&](int at, const double(&k)[4]) __attribute__((always_inline)) {
+      stAt[wave][lane] = at;
+#pragma unroll
+      for (int c = 0; c < 4; c++) stK[wave][c][lane] = k[c];
+      wave_sync();
+      const int c = lane & 3;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int src = (lane >> 2) + 16 * r;
+        const int a = stAt[wave][src];
+        if (a >= 0) {
+          double *blk = args.values + a + c;
+          if (ATOMIC) unsafeAtomicAdd(blk, stK[wave][c][src]);
+          else *blk += stK[wave][c][src];
+        }
+      }
+      wave_sync();  // the next row restages into the same slots
+    };
+    stage_stiffness_rows<P, T, JAC, true>(W, gref, args.mat, el, idx, valid, N, N, Rx, Ry, where, emitRows);
   } else {
     stage_stiffness<P, T, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, where, emit);
   }
@@ -752,7 +773,7 @@ __device__ __forceinline__ void gather_level2(const MeshJob &J, int idx, Gather<
 // Software-pipelined over quadrature points like stage_stiffness: iteration q publishes point q+1's
 // weighted gradient of node b and issues the reads of the chunk's rows, then accumulates point q from
 // values read one iteration earlier.
-template <int P, typename T, bool JAC, int CH, typename Where, typename Emit>
+template <int P, typename T, bool JAC, int CH, bool ALL_LANES, typename Where, typename Emit>
 __device__ __forceinline__ void rows_chunk(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el, int b,
                                            bool valid, int count, int base, bool first, double &Rx, double &Ry,
                                            Where &&where, Emit &&emit) {
@@ -827,10 +848,10 @@ __device__ __forceinline__ void rows_chunk(WaveLDS<P, T> &W, const double2 *gref
     Rx += rx;
     Ry += ry;
   }
-  if (JAC && valid) {
+  if (JAC && (ALL_LANES || valid)) {
     int at[CH];
 #pragma unroll
-    for (int j = 0; j < CH; j++) at[j] = base + j < count ? where(arow[j], b) : -1;
+    for (int j = 0; j < CH; j++) at[j] = valid && base + j < count ? where(arow[j], b) : -1;
 #pragma unroll
     for (int j = 0; j < CH; j++) {
       if (base + j >= count) break;
@@ -849,8 +870,10 @@ __device__ __forceinline__ void rows_chunk(WaveLDS<P, T> &W, const double2 *gref
 // K(a, b) only for the rows a this part owns, W.orow[el][0..count), in chunks of 8 rows (4 in FP64; maxCount,
 // the largest count over the wavefront's elements, sets the number of chunks). Over all parts visiting an
 // element this forms each row of K_e once, instead of the whole symmetric K_e per visit. The residual of
-// node b is accumulated in the first chunk. where and emit as for stage_stiffness.
-template <int P, typename T, bool JAC, typename Where, typename Emit>
+// node b is accumulated in the first chunk. where and emit as for stage_stiffness, except that emit gets a
+// negative destination for rows it must skip. With ALL_LANES every lane calls emit for every row of a
+// chunk, in step (count must then be the same for all lanes), so emit may exchange values across lanes.
+template <int P, typename T, bool JAC, bool ALL_LANES, typename Where, typename Emit>
 __device__ __forceinline__ void stage_stiffness_rows(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el,
                                                      int b, bool valid, int count, int maxCount, double &Rx,
                                                      double &Ry, Where &&where, Emit &&emit) {
@@ -860,7 +883,7 @@ __device__ __forceinline__ void stage_stiffness_rows(WaveLDS<P, T> &W, const dou
   const int chunks = JAC ? uniform(max((maxCount + CH - 1) / CH, 1)) : 1;
 #pragma unroll 1
   for (int c = 0; c < chunks; c++) {
-    rows_chunk<P, T, JAC, CH>(W, gref, m, el, b, valid, count, c * CH, c == 0, Rx, Ry, where, emit);
+    rows_chunk<P, T, JAC, CH, ALL_LANES>(W, gref, m, el, b, valid, count, c * CH, c == 0, Rx, Ry, where, emit);
   }
 }
 
@@ -1100,7 +1123,7 @@ __device__ __forceinline__ void batch_stiffness(const Material &mat, WaveLDS<P, 
   if constexpr (P >= 3 && JAC && std::is_same<T, float>::value) {
     stage_stiffness_mfma<P, MAXB, DET>(W, gref, mat, el, idx, ownMask, Rx, Ry, acc);
   } else if constexpr (P >= 3) {
-    stage_stiffness_rows<P, T, JAC>(W, gref, mat, el, idx, valid, count, maxCount, Rx, Ry, where, emit);
+    stage_stiffness_rows<P, T, JAC, false>(W, gref, mat, el, idx, valid, count, maxCount, Rx, Ry, where, emit);
   } else {
     stage_stiffness<P, T, JAC>(W, gref, mat, el, idx, valid, Rx, Ry, where, emit);
   }
