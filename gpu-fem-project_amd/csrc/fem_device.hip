@@ -582,8 +582,8 @@ __device__ __forceinline__ void stage_geometry_nl(WaveLDSNL<P> &W, const Basis1D
 
 // Lane (el, b): residual of node b and (JAC) the blocks K(a, b) of every row a, in chunks of CH rows; the
 // gradients of the chunk's rows come through the per-wave LDS one quadrature point at a time. All lanes of
-// the wavefront must call this; `valid` masks the lane's writes and emits. where/emit as for
-// stage_stiffness.
+// the wavefront must call this; `valid` masks the lane's writes. Every lane calls emit(at, k) for every row,
+// in step, with a negative destination when it has none (so emit may exchange values across lanes).
 template <int P, bool JAC, typename Where, typename Emit>
 __device__ __forceinline__ void stage_stiffness_nl(WaveLDSNL<P> &W, const double2 *gref, const Material &m, int el,
                                                    int b, bool valid, double &Rx, double &Ry, Where &&where,
@@ -641,10 +641,10 @@ __device__ __forceinline__ void stage_stiffness_nl(WaveLDSNL<P> &W, const double
         wave_sync();  // the next point republishes into the same slots
       }
     }
-    if (JAC && valid) {
+    if (JAC) {  // every lane calls emit for every row, in step (negative destination: none)
       int at[CH];
 #pragma unroll
-      for (int j = 0; j < CH; j++) at[j] = c0 + j < N ? where(c0 + j, b) : -1;
+      for (int j = 0; j < CH; j++) at[j] = valid && c0 + j < N ? where(c0 + j, b) : -1;
 #pragma unroll
       for (int j = 0; j < CH; j++) {
         if (c0 + j >= N) break;
@@ -664,6 +664,8 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel_nl(const ElemArgs a
   __shared__ WaveLDSNL<P> wl[WAVES];
   __shared__ Basis1D basis;
   __shared__ double2 gref[O::Q * O::N];
+  __shared__ double stK[JAC ? WAVES : 1][4][64];  // Jacobian emission staging, as in element_kernel
+  __shared__ int stAt[JAC ? WAVES : 1][64];
   copy_basis(basis, args.basis);
   fill_gref<P>(gref, args.basis);
   __syncthreads();
@@ -686,12 +688,22 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel_nl(const ElemArgs a
   const int *mel = args.map + int64_t(eid) * N * N;
   double Rx, Ry;
   auto where = [&](int row, int col) __attribute__((always_inline)) { return mel[row * N + col]; };
-  stage_stiffness_nl<P, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, where,
-                             [&](int at, const double(&k)[4]) __attribute__((always_inline)) {
-                               double *blk = args.values + at;
+  // Each add instruction covers the four components of 16 blocks (lanes 4i..4i+3 take block i).
+  auto emit = [&](int at, const double(&k)[4]) __attribute__((always_inline)) {
+    stAt[wave][lane] = at;
 #pragma unroll
-                               for (int c = 0; c < 4; c++) unsafeAtomicAdd(blk + c, k[c]);
-                             });
+    for (int c = 0; c < 4; c++) stK[wave][c][lane] = k[c];
+    wave_sync();
+    const int c = lane & 3;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int src = (lane >> 2) + 16 * r;
+      const int a = stAt[wave][src];
+      if (a >= 0) unsafeAtomicAdd(args.values + a + c, stK[wave][c][src]);
+    }
+    wave_sync();
+  };
+  stage_stiffness_nl<P, JAC>(W, gref, args.mat, el, idx, valid, Rx, Ry, where, emit);
   if (RES && valid) {
     double *r = args.residual + 2 * int64_t(node);
     unsafeAtomicAdd(r, Rx);
