@@ -415,9 +415,13 @@ struct ElemArgs {
   double *values;
   Basis1D basis;
   Material mat;
+  // Element-once traversal of a resident plan (PLAN kernels): conn is [E][N] (connPtr unused), and a block's
+  // destination is 4 * (rowp[node a] + rank of column b in row a) instead of a map entry.
+  const uint32_t *rank;  // [E][N][RPAD] bytes, as MeshJob::rank
+  const int *rowp;       // [nodes + 1] BSR row starts
 };
 
-template <int P, typename T, int WAVES, bool RES, bool JAC, bool ATOMIC>
+template <int P, typename T, int WAVES, bool RES, bool JAC, bool ATOMIC, bool PLAN = false>
 __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args) {
   using O = Ord<P>;
   constexpr int N = O::N;
@@ -438,16 +442,26 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   int eid = 0, node = 0;
   if (valid) {
     eid = args.elemList ? args.elemList[slot] : args.first + int(slot);
-    node = args.conn[args.connPtr[eid] + idx];
+    node = PLAN ? args.conn[int64_t(eid) * N + idx] : args.conn[args.connPtr[eid] + idx];
     store_nodal(W, el, idx, *reinterpret_cast<const double2 *>(args.coords + 2 * int64_t(node)),
                 *reinterpret_cast<const double2 *>(args.states + 2 * int64_t(node)));
+    if constexpr (PLAN) {
+      constexpr int RW = O::RPAD / 4;
+      const gptr<uint32_t> r = gmem(args.rank) + (int64_t(eid) * N + idx) * RW;
+#pragma unroll
+      for (int i = 0; i < RW; i++) W.rk[el][idx][i] = r[i];
+      W.obase[el][idx] = args.rowp[node];
+    }
   }
   wave_sync();
   if (valid) stage_geometry<P, T>(W, basis, args.mat, el, idx);
   wave_sync();
   const int *mel = args.map + int64_t(eid) * N * N;
   double Rx, Ry;
-  auto where = [&](int row, int col) __attribute__((always_inline)) { return mel[row * N + col]; };
+  auto where = [&](int row, int col) __attribute__((always_inline)) {
+    if constexpr (PLAN) return 4 * (W.obase[el][row] + rank_of(W, el, row, col));
+    else return mel[row * N + col];
+  };
   auto emit = [&](int at, const double(&k)[4]) __attribute__((always_inline)) {
     double *blk = args.values + at;
     if (ATOMIC) {
@@ -1900,6 +1914,24 @@ void launch_element_order(int order, const ElemArgs &a, int precision, hipStream
   }
 }
 
+// Fills n doubles with zero (a kernel, so that it can be recorded in a graph).
+__global__ __launch_bounds__(256) void zero_fill(double *p, int64_t n) {
+  const int64_t stride = int64_t(gridDim.x) * 256;
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) p[i] = 0;
+}
+
+void launch_zero_fill(double *p, int64_t n, hipStream_t stream) {
+  if (n > 0) zero_fill<<<unsigned(std::min<int64_t>((n + 255) / 256, 4096)), 256, 0, stream>>>(p, n);
+}
+
+template <int P, bool RES>
+void launch_element_plan(const ElemArgs &a, hipStream_t stream) {
+  constexpr int W = elem_waves(P);
+  constexpr int per = W * Ord<P>::EPW;
+  const int grid = (a.count + per - 1) / per;
+  if (grid > 0) element_kernel<P, double, W, RES, true, true, true><<<grid, W * 64, 0, stream>>>(a);
+}
+
 template <int P, bool RES, bool JAC>
 void launch_element_nl(const ElemArgs &a, hipStream_t stream) {
   constexpr int W = elem_waves(P);
@@ -2055,8 +2087,8 @@ int upload(const std::vector<TPtr> &h, TPtr **d, int64_t &bytes) {
 }
 
 void free_plan_arrays(fem_plan *p) {
-  void *ptrs[] = {p->dVisitConn, p->dVisitEid, p->dNodeLrow, p->dRows,     p->dRowLbase, p->dRuns,
-                  p->dColourOff, p->dPartDesc, p->dRank,     p->dJob,      p->dJobStart};
+  void *ptrs[] = {p->dVisitConn, p->dVisitEid, p->dNodeLrow, p->dRows,     p->dRowLbase, p->dRuns,   p->dColourOff,
+                  p->dPartDesc,  p->dRank,     p->dJob,      p->dJobStart, p->dConn,     p->dRowp};
   for (void *q : ptrs) {
     if (q) (void)hipFree(q);
   }
@@ -2243,6 +2275,21 @@ int create_plan(int order, int numNodes, int numElements, const int *connPtr, co
     delete p;
     return rc;
   }
+  // Orders 3-4: FP64 Jacobian assemblies traverse the elements once with global atomics (faster there than
+  // the owner-computes parts, which visit each element 3-4 times), from the connectivity and row starts.
+  const bool elementOnce = order >= 3 && !(flags & (FEM_PLAN_DETERMINISTIC | FEM_PLAN_RESIDUAL_ONLY)) &&
+                           4 * h.numBlocks <= INT32_MAX;
+  if (elementOnce) {
+    const int N = (order + 1) * (order + 1);
+    std::vector<int> c(size_t(numElements) * N);
+    for (int e = 0; e < numElements; e++) std::copy(conn + connPtr[e], conn + connPtr[e] + N, c.begin() + size_t(e) * N);
+    const std::vector<int> r(rowp, rowp + numNodes + 1);
+    if ((rc = upload(c, &p->dConn, bytes)) || (rc = upload(r, &p->dRowp, bytes))) {
+      free_plan_arrays(p);
+      delete p;
+      return rc;
+    }
+  }
   if (hipMalloc(reinterpret_cast<void **>(&p->dJob), sizeof(MeshJob)) != hipSuccess) {
     free_plan_arrays(p);
     delete p;
@@ -2382,6 +2429,33 @@ int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCo
   L.rOnlyParts = rOnly;
   L.stream = plan_streams(plan);
   L.streamGrid = stream_grid(plan->numCUs);
+  if (plan->dConn && precision == FEM_PREC_FP64 && matEntries) {
+    // Element-once traversal (orders 3-4, FP64): zero the outputs, then add every element's blocks with
+    // coalesced global atomics.
+    ElemArgs a{};
+    a.conn = plan->dConn;
+    a.count = plan->numElements;
+    a.states = states;
+    a.coords = nodeCoords;
+    a.residual = residual;
+    a.values = matEntries;
+    a.rank = reinterpret_cast<const uint32_t *>(plan->dRank);
+    a.rowp = plan->dRowp;
+    a.basis = L.args.basis;
+    a.mat = m;
+    return timed(s, elapsedSeconds, [&]() {
+      launch_zero_fill(matEntries, 4 * plan->numBlocks, s);
+      if (residual) launch_zero_fill(residual, 2 * int64_t(plan->numNodes), s);
+      if (plan->order == 3) {
+        if (residual) launch_element_plan<3, true>(a, s);
+        else launch_element_plan<3, false>(a, s);
+      } else {
+        if (residual) launch_element_plan<4, true>(a, s);
+        else launch_element_plan<4, false>(a, s);
+      }
+      return FEM_OK;
+    });
+  }
   return timed(s, elapsedSeconds, [&]() {
     launch_owner_any(plan->order, L, s);
     return FEM_OK;

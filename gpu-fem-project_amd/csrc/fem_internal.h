@@ -124,6 +124,8 @@ struct fem_plan {
   int *dPartDesc = nullptr;
   uint8_t *dRank = nullptr;
   int *dJobStart = nullptr;  // {0, numParts}
+  int *dConn = nullptr;      // [E][N] (orders 3-4, FP64 Jacobian plans: element-once traversal)
+  int *dRowp = nullptr;      // [nodes + 1] BSR row starts (same plans)
   double partScale = 1.0;    // part budgets relative to the kernel's (FEM_PLAN_AUTOTUNE picks it)
   double tunedSeconds = 0;   // measured assembly time of the chosen build (autotuned plans)
   void *dJob = nullptr;      // device job record of the last assembly
