@@ -65,12 +65,13 @@ def make_workload(F, name):
 
 
 def plan_kernel_name(args, order):
-    """The kernel(s) a plan assembly launches: owner-computes parts, or (orders 3-4 in FP64) a zero-fill and
-    one element-once traversal with global atomics."""
+    """The kernel(s) a plan assembly launches: owner-computes parts (orders 1-2), or (orders 3-4) one
+    element-once traversal that stores exclusive blocks and stages shared ones, then the gathers of the
+    staged sums."""
+    if order >= 3:
+        return "element_kernel+gather_shared"
     if args.path == "plan-det":
         return "owner_kernel"
-    if order >= 3 and args.precision == "fp64":
-        return "zero_fill+element_kernel"
     return "owner_stream_kernel"
 
 

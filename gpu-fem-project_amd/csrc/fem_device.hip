@@ -415,13 +415,18 @@ struct ElemArgs {
   double *values;
   Basis1D basis;
   Material mat;
-  // Element-once traversal of a resident plan (PLAN kernels): conn is [E][N] (connPtr unused), and a block's
-  // destination is 4 * (rowp[node a] + rank of column b in row a) instead of a map entry.
-  const uint32_t *rank;  // [E][N][RPAD] bytes, as MeshJob::rank
-  const int *rowp;       // [nodes + 1] BSR row starts
+  // Element-once assembly of a resident plan (SPLIT kernels): conn is [E][N] (connPtr unused) and map holds
+  // [E][N][N] block codes, row-major: a code below numBlocks is the BSR index of a block this element alone
+  // forms (stored directly), else numBlocks + the block's slot in stage (summed by gather_shared). rcode
+  // [E][N] codes each node's residual likewise, against numNodes and rstage. Every store is plain.
+  double *stage;
+  const int *rcode;
+  double *rstage;
+  int numBlocks;
+  int numNodes;
 };
 
-template <int P, typename T, int WAVES, bool RES, bool JAC, bool ATOMIC, bool PLAN = false>
+template <int P, typename T, int WAVES, bool RES, bool JAC, bool ATOMIC, bool SPLIT = false>
 __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args) {
   using O = Ord<P>;
   constexpr int N = O::N;
@@ -442,26 +447,16 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   int eid = 0, node = 0;
   if (valid) {
     eid = args.elemList ? args.elemList[slot] : args.first + int(slot);
-    node = PLAN ? args.conn[int64_t(eid) * N + idx] : args.conn[args.connPtr[eid] + idx];
+    node = SPLIT ? args.conn[int64_t(eid) * N + idx] : args.conn[args.connPtr[eid] + idx];
     store_nodal(W, el, idx, *reinterpret_cast<const double2 *>(args.coords + 2 * int64_t(node)),
                 *reinterpret_cast<const double2 *>(args.states + 2 * int64_t(node)));
-    if constexpr (PLAN) {
-      constexpr int RW = O::RPAD / 4;
-      const gptr<uint32_t> r = gmem(args.rank) + (int64_t(eid) * N + idx) * RW;
-#pragma unroll
-      for (int i = 0; i < RW; i++) W.rk[el][idx][i] = r[i];
-      W.obase[el][idx] = args.rowp[node];
-    }
   }
   wave_sync();
   if (valid) stage_geometry<P, T>(W, basis, args.mat, el, idx);
   wave_sync();
   const int *mel = args.map + int64_t(eid) * N * N;
   double Rx, Ry;
-  auto where = [&](int row, int col) __attribute__((always_inline)) {
-    if constexpr (PLAN) return 4 * (W.obase[el][row] + rank_of(W, el, row, col));
-    else return mel[row * N + col];
-  };
+  auto where = [&](int row, int col) __attribute__((always_inline)) { return mel[row * N + col]; };
   auto emit = [&](int at, const double(&k)[4]) __attribute__((always_inline)) {
     double *blk = args.values + at;
     if (ATOMIC) {
@@ -489,9 +484,15 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
         const int src = (lane >> 2) + 16 * r;
         const int a = stAt[wave][src];
         if (a >= 0) {
-          double *blk = args.values + a + c;
-          if (ATOMIC) unsafeAtomicAdd(blk, stK[wave][c][src]);
-          else *blk += stK[wave][c][src];
+          if constexpr (SPLIT) {
+            double *blk = a < args.numBlocks ? args.values + 4 * int64_t(a)
+                                             : args.stage + 4 * (int64_t(a) - args.numBlocks);
+            __builtin_nontemporal_store(stK[wave][c][src], blk + c);
+          } else {
+            double *blk = args.values + a + c;
+            if (ATOMIC) unsafeAtomicAdd(blk, stK[wave][c][src]);
+            else *blk += stK[wave][c][src];
+          }
         }
       }
       wave_sync();  // the next row restages into the same slots
@@ -502,7 +503,11 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   }
   if (RES && valid) {
     double *r = args.residual + 2 * int64_t(node);
-    if (ATOMIC) {
+    if constexpr (SPLIT) {
+      const int rc = args.rcode[int64_t(eid) * N + idx];
+      if (rc >= args.numNodes) r = args.rstage + 2 * (int64_t(rc) - args.numNodes);
+      *reinterpret_cast<double2 *>(r) = make_double2(Rx, Ry);
+    } else if (ATOMIC) {
       unsafeAtomicAdd(r, double(Rx));
       unsafeAtomicAdd(r + 1, double(Ry));
     } else {
@@ -1914,22 +1919,52 @@ void launch_element_order(int order, const ElemArgs &a, int precision, hipStream
   }
 }
 
-// Fills n doubles with zero (a kernel, so that it can be recorded in a graph).
-__global__ __launch_bounds__(256) void zero_fill(double *p, int64_t n) {
-  const int64_t stride = int64_t(gridDim.x) * 256;
-  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) p[i] = 0;
-}
-
-void launch_zero_fill(double *p, int64_t n, hipStream_t stream) {
-  if (n > 0) zero_fill<<<unsigned(std::min<int64_t>((n + 255) / 256, 4096)), 256, 0, stream>>>(p, n);
-}
-
-template <int P, bool RES>
-void launch_element_plan(const ElemArgs &a, hipStream_t stream) {
+template <int P, typename T, bool RES>
+void launch_element_split(const ElemArgs &a, hipStream_t stream) {
   constexpr int W = elem_waves(P);
   constexpr int per = W * Ord<P>::EPW;
   const int grid = (a.count + per - 1) / per;
-  if (grid > 0) element_kernel<P, double, W, RES, true, true, true><<<grid, W * 64, 0, stream>>>(a);
+  if (grid > 0) element_kernel<P, T, W, RES, true, false, true><<<grid, W * 64, 0, stream>>>(a);
+}
+
+// Second pass of the element-once assembly: output dst[i] (C doubles from C * dst[i]) is the sum of the
+// staged contributions in slots ptr[i] .. ptr[i+1]-1, in slot order; zero if there are none. Lane t covers
+// component t % C of output t / C, so a wavefront reads consecutive slots.
+template <int C>
+__global__ __launch_bounds__(256) void gather_shared(const int *dst, const int *ptr, const double *stage,
+                                                     double *out, int n) {
+  const int64_t total = int64_t(n) * C, stride = int64_t(gridDim.x) * 256;
+  for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total; t += stride) {
+    const int64_t i = t / C;
+    const int c = int(t % C);
+    const int k0 = ptr[i], k1 = ptr[i + 1];
+    double sum = 0;
+    for (int k = k0; k < k1; k++) sum += __builtin_nontemporal_load(stage + int64_t(C) * k + c);
+    __builtin_nontemporal_store(sum, out + int64_t(C) * dst[i] + c);
+  }
+}
+
+template <int C>
+void launch_gather_shared(const int *dst, const int *ptr, const double *stage, double *out, int n,
+                          hipStream_t stream) {
+  const int64_t blocks = (int64_t(n) * C + 255) / 256;
+  if (n > 0) gather_shared<C><<<unsigned(std::min<int64_t>(blocks, 1 << 20)), 256, 0, stream>>>(dst, ptr, stage, out, n);
+}
+
+// Element-once assembly of a plan with split tables: the element kernel stores exclusive outputs and stages
+// the rest, then the gathers store the staged sums.
+template <int P>
+void launch_split(const fem_plan *plan, ElemArgs a, int precision, hipStream_t s) {
+  const bool res = a.residual != nullptr;
+  if (precision == FEM_PREC_MIXED) {
+    if (res) launch_element_split<P, float, true>(a, s);
+    else launch_element_split<P, float, false>(a, s);
+  } else {
+    if (res) launch_element_split<P, double, true>(a, s);
+    else launch_element_split<P, double, false>(a, s);
+  }
+  launch_gather_shared<4>(plan->dGdst, plan->dGptr, plan->dStage, a.values, plan->numGather, s);
+  if (res) launch_gather_shared<2>(plan->dRdst, plan->dRptr, plan->dRstage, a.residual, plan->numRgather, s);
 }
 
 template <int P, bool RES, bool JAC>
@@ -2086,9 +2121,17 @@ int upload(const std::vector<TPtr> &h, TPtr **d, int64_t &bytes) {
   return FEM_OK;
 }
 
+int alloc_doubles(int64_t n, double **d, int64_t &bytes) {
+  const size_t sz = size_t(std::max<int64_t>(n, 1)) * sizeof(double);
+  FEM_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(d), sz));
+  bytes += int64_t(sz);
+  return FEM_OK;
+}
+
 void free_plan_arrays(fem_plan *p) {
-  void *ptrs[] = {p->dVisitConn, p->dVisitEid, p->dNodeLrow, p->dRows,     p->dRowLbase, p->dRuns,   p->dColourOff,
-                  p->dPartDesc,  p->dRank,     p->dJob,      p->dJobStart, p->dConn,     p->dRowp};
+  void *ptrs[] = {p->dVisitConn, p->dVisitEid, p->dNodeLrow, p->dRows,  p->dRowLbase, p->dRuns,  p->dColourOff,
+                  p->dPartDesc,  p->dRank,     p->dJob,      p->dJobStart, p->dConn,   p->dCode,  p->dRcode,
+                  p->dGdst,      p->dGptr,     p->dRdst,     p->dRptr,   p->dStage,    p->dRstage};
   for (void *q : ptrs) {
     if (q) (void)hipFree(q);
   }
@@ -2275,19 +2318,30 @@ int create_plan(int order, int numNodes, int numElements, const int *connPtr, co
     delete p;
     return rc;
   }
-  // Orders 3-4: FP64 Jacobian assemblies traverse the elements once with global atomics (faster there than
-  // the owner-computes parts, which visit each element 3-4 times), from the connectivity and row starts.
-  const bool elementOnce = order >= 3 && !(flags & (FEM_PLAN_DETERMINISTIC | FEM_PLAN_RESIDUAL_ONLY)) &&
-                           4 * h.numBlocks <= INT32_MAX;
-  if (elementOnce) {
+  // Orders 3-4: Jacobian assemblies traverse the elements once (the owner-computes parts visit each element
+  // 3-4 times there). Most blocks of an element's K_e are formed by that element alone and are stored
+  // directly; the rest are staged and summed by a gather pass (SplitHost). Without room for int codes the
+  // plan keeps the owner-computes path.
+  if (order >= 3 && !(flags & FEM_PLAN_RESIDUAL_ONLY)) {
     const int N = (order + 1) * (order + 1);
     std::vector<int> c(size_t(numElements) * N);
     for (int e = 0; e < numElements; e++) std::copy(conn + connPtr[e], conn + connPtr[e] + N, c.begin() + size_t(e) * N);
-    const std::vector<int> r(rowp, rowp + numNodes + 1);
-    if ((rc = upload(c, &p->dConn, bytes)) || (rc = upload(r, &p->dRowp, bytes))) {
-      free_plan_arrays(p);
-      delete p;
-      return rc;
+    SplitHost sp;
+    if (build_split_host(order, numNodes, numElements, c.data(), rowp, h.rank, h.numBlocks, sp) == FEM_OK) {
+      h = PlanHost();  // the owner tables are on the device; release their host copies first
+      if ((rc = upload(c, &p->dConn, bytes)) || (rc = upload(sp.code, &p->dCode, bytes)) ||
+          (rc = upload(sp.rcode, &p->dRcode, bytes)) || (rc = upload(sp.dst, &p->dGdst, bytes)) ||
+          (rc = upload(sp.ptr, &p->dGptr, bytes)) || (rc = upload(sp.rdst, &p->dRdst, bytes)) ||
+          (rc = upload(sp.rptr, &p->dRptr, bytes)) || (rc = alloc_doubles(4 * sp.slots, &p->dStage, bytes)) ||
+          (rc = alloc_doubles(2 * sp.rslots, &p->dRstage, bytes))) {
+        free_plan_arrays(p);
+        delete p;
+        return rc;
+      }
+      p->numGather = int(sp.dst.size());
+      p->numRgather = int(sp.rdst.size());
+    } else {
+      clear_error();
     }
   }
   if (hipMalloc(reinterpret_cast<void **>(&p->dJob), sizeof(MeshJob)) != hipSuccess) {
@@ -2344,7 +2398,10 @@ int fem_plan_create(int order, int numNodes, int numElements, const int *connPtr
   FEM_HIP_CHECK(hipMemset(tb.states, 0, size_t(numNodes) * 16));
   fem_plan *best = nullptr;
   double bestSeconds = 0;
-  for (const double scale : {1.0, 0.5, 0.25}) {
+  // Orders 3-4 assemble their Jacobians element by element, independent of the part size; only full-size
+  // parts are built there.
+  const std::vector<double> scales = order >= 3 ? std::vector<double>{1.0} : std::vector<double>{1.0, 0.5, 0.25};
+  for (const double scale : scales) {
     fem_plan *cand = nullptr;
     int rc = create_plan(order, numNodes, numElements, connPtr, conn, rowp, cols, nodeCoordsHost, flags, scale,
                          &cand);
@@ -2429,30 +2486,27 @@ int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCo
   L.rOnlyParts = rOnly;
   L.stream = plan_streams(plan);
   L.streamGrid = stream_grid(plan->numCUs);
-  if (plan->dConn && precision == FEM_PREC_FP64 && matEntries) {
-    // Element-once traversal (orders 3-4, FP64): zero the outputs, then add every element's blocks with
-    // coalesced global atomics.
+  if (plan->dConn && matEntries) {
+    // Element-once assembly (orders 3-4): exclusive outputs stored by their element, the rest staged and
+    // gathered. Deterministic, so it also serves FEM_PLAN_DETERMINISTIC plans.
     ElemArgs a{};
     a.conn = plan->dConn;
     a.count = plan->numElements;
     a.states = states;
     a.coords = nodeCoords;
+    a.map = plan->dCode;
     a.residual = residual;
     a.values = matEntries;
-    a.rank = reinterpret_cast<const uint32_t *>(plan->dRank);
-    a.rowp = plan->dRowp;
+    a.stage = plan->dStage;
+    a.rcode = plan->dRcode;
+    a.rstage = plan->dRstage;
+    a.numBlocks = int(plan->numBlocks);
+    a.numNodes = plan->numNodes;
     a.basis = L.args.basis;
     a.mat = m;
     return timed(s, elapsedSeconds, [&]() {
-      launch_zero_fill(matEntries, 4 * plan->numBlocks, s);
-      if (residual) launch_zero_fill(residual, 2 * int64_t(plan->numNodes), s);
-      if (plan->order == 3) {
-        if (residual) launch_element_plan<3, true>(a, s);
-        else launch_element_plan<3, false>(a, s);
-      } else {
-        if (residual) launch_element_plan<4, true>(a, s);
-        else launch_element_plan<4, false>(a, s);
-      }
+      if (plan->order == 3) launch_split<3>(plan, a, precision, s);
+      else launch_split<4>(plan, a, precision, s);
       return FEM_OK;
     });
   }

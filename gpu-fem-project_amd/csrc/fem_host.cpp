@@ -139,7 +139,6 @@ void row_columns(int n, int npe, const int *connPtr, const int *conn, const std:
   tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
 }
 
-// Position of `col` in cols[b, e) (sorted); -1 if absent.
 // Bijective 32-bit mix (xor-shift-multiply), used as a fixed pseudo-random sort key.
 inline uint32_t scramble(uint32_t v) {
   v ^= v >> 16;
@@ -149,6 +148,7 @@ inline uint32_t scramble(uint32_t v) {
   return v ^ (v >> 16);
 }
 
+// Position of `col` in cols[b, e) (sorted); -1 if absent.
 inline int find_sorted(const int *cols, int b, int e, int col) {
   const int *p = std::lower_bound(cols + b, cols + e, col);
   return (p != cols + e && *p == col) ? int(p - cols) : -1;
@@ -403,6 +403,64 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
     }
   });
   return FEM_OK;
+}
+
+namespace {
+
+// Shared by the matrix and residual halves of build_split_host: from the contributions' destinations
+// (dest(i), i = 0..n-1, each in [0, outputs)), writes code[i] (the destination if it is its output's only
+// contribution, else outputs + a slot) and the gather list of the other outputs. Slots are grouped by output,
+// ascending, and within an output ordered by contribution index.
+template <typename Dest>
+int split_outputs(int64_t n, int64_t outputs, Dest &&dest, std::vector<int> &code, std::vector<int> &dst,
+                  std::vector<int> &ptr, int64_t &slots) {
+  std::vector<int> next(size_t(outputs), 0);  // first a count per output, then its next free slot
+  for (int64_t i = 0; i < n; i++) next[size_t(dest(i))]++;
+  int64_t running = 0;
+  ptr.assign(1, 0);
+  dst.clear();
+  for (int64_t d = 0; d < outputs; d++) {
+    const int c = next[size_t(d)];
+    if (c == 1) {
+      next[size_t(d)] = -1;  // exclusive
+      continue;
+    }
+    next[size_t(d)] = int(running);
+    running += c;
+    if (outputs + running > INT32_MAX) {
+      return set_error(FEM_ERR_UNSUPPORTED, "%lld outputs and %lld staged contributions exceed int codes",
+                       (long long)outputs, (long long)running);
+    }
+    dst.push_back(int(d));
+    ptr.push_back(int(running));
+  }
+  slots = running;
+  code.resize(size_t(n));
+  for (int64_t i = 0; i < n; i++) {
+    const int d = dest(i);
+    int &s = next[size_t(d)];
+    code[size_t(i)] = s < 0 ? d : int(outputs) + s++;
+  }
+  return FEM_OK;
+}
+
+}  // namespace
+
+int build_split_host(int order, int numNodes, int numElements, const int *conn, const int *rowp,
+                     const std::vector<uint8_t> &rank, int64_t numBlocks, SplitHost &out) {
+  const int N = nodes_per_element(order);
+  const int RP = rank_pad(order);
+  out = SplitHost();
+  // block (e, a, b), row-major: rows of node conn[e,a], column rank rank[(e*N + b)*RP + a]
+  auto blockDest = [&](int64_t i) {
+    const int64_t e = i / (N * N);
+    const int a = int(i / N % N), b = int(i % N);
+    return rowp[conn[e * N + a]] + int(rank[size_t((e * N + b) * RP + a)]);
+  };
+  int rc = split_outputs(int64_t(numElements) * N * N, numBlocks, blockDest, out.code, out.dst, out.ptr, out.slots);
+  if (rc) return rc;
+  auto nodeDest = [&](int64_t i) { return conn[i]; };
+  return split_outputs(int64_t(numElements) * N, numNodes, nodeDest, out.rcode, out.rdst, out.rptr, out.rslots);
 }
 
 }  // namespace fem

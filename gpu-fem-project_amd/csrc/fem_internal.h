@@ -94,6 +94,28 @@ struct PlanHost {
   int maxPartVisits = 0;
 };
 
+// Element-once assembly tables (orders 3-4). Every output (a BSR block, or a node's residual pair) is either
+// exclusive, formed by exactly one (element, node[, node]) contribution, or not. An exclusive output is
+// stored by its one contributor; the contributions to any other output go to slots of a staging array,
+// grouped by output, and a gather pass stores each such output as the sum of its slots in slot order (zero
+// when no element touches it). Every output is written exactly once and the result is bitwise
+// reproducible.
+struct SplitHost {
+  std::vector<int> code;   // [E][N][N], row-major (a, b): block of rows conn[e,a] x column conn[e,b]; below
+                           // numBlocks the block's BSR index (exclusive), else numBlocks + staging slot
+  std::vector<int> rcode;  // [E][N]: conn[e,a] if the node's residual is exclusive, else numNodes + slot
+  std::vector<int> dst;    // BSR indices of the non-exclusive blocks, ascending
+  std::vector<int> ptr;    // [dst.size() + 1] slot range of each
+  std::vector<int> rdst;   // non-exclusive residual nodes, ascending
+  std::vector<int> rptr;   // [rdst.size() + 1]
+  int64_t slots = 0, rslots = 0;
+};
+
+// Builds the element-once tables from the connectivity ([E][N], element-major), the BSR row starts and a
+// plan's column ranks. Returns FEM_ERR_UNSUPPORTED if the codes would not fit in an int.
+int build_split_host(int order, int numNodes, int numElements, const int *conn, const int *rowp,
+                     const std::vector<uint8_t> &rank, int64_t numBlocks, SplitHost &out);
+
 // partScale (0, 1] scales the part budgets (blocks, rows, visits) below the kernel's maxima: smaller
 // parts give small meshes more workgroups at the price of more elements recomputed on part boundaries.
 int build_plan_host(int order, int numNodes, int numElements, const int *connPtr, const int *conn,
@@ -124,8 +146,15 @@ struct fem_plan {
   int *dPartDesc = nullptr;
   uint8_t *dRank = nullptr;
   int *dJobStart = nullptr;  // {0, numParts}
-  int *dConn = nullptr;      // [E][N] (orders 3-4, FP64 Jacobian plans: element-once traversal)
-  int *dRowp = nullptr;      // [nodes + 1] BSR row starts (same plans)
+  // Element-once assembly of Jacobian plans of orders 3-4 (SplitHost): connectivity, codes, gather lists and
+  // staging arrays. dConn is NULL when the plan has none.
+  int *dConn = nullptr;      // [E][N]
+  int *dCode = nullptr;      // [E][N][N]
+  int *dRcode = nullptr;     // [E][N]
+  int *dGdst = nullptr, *dGptr = nullptr, *dRdst = nullptr, *dRptr = nullptr;
+  double *dStage = nullptr;  // [4 * slots]
+  double *dRstage = nullptr; // [2 * rslots]
+  int numGather = 0, numRgather = 0;
   double partScale = 1.0;    // part budgets relative to the kernel's (FEM_PLAN_AUTOTUNE picks it)
   double tunedSeconds = 0;   // measured assembly time of the chosen build (autotuned plans)
   void *dJob = nullptr;      // device job record of the last assembly
