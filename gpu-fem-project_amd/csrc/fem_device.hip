@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <type_traits>
+#include <utility>
 #include <cstring>
 #include <vector>
 
@@ -255,9 +256,8 @@ struct PointGeo {
   double2 g, ia, ib, sc, sd;
 };
 
-template <int P, typename T>
-__device__ __forceinline__ void load_point(const WaveLDS<P, T> &W, const double2 *gref, int el, int b, int q,
-                                           PointGeo &G) {
+template <int P, typename T, typename WL>
+__device__ __forceinline__ void load_point(const WL &W, const double2 *gref, int el, int b, int q, PointGeo &G) {
   const double *g = &W.geo[el][8 * q];
   G.g = gref[q * Ord<P>::N + b];
   G.ia = *reinterpret_cast<const double2 *>(g + 0);
@@ -515,6 +515,242 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
       r[1] += double(Ry);
     }
   }
+}
+
+// ---- element-once assembly of a plan with split tables, orders 3-4 -----------------------------------
+// K_e is formed in tiles of 4 x 4 nodes, one tile per lane, upper tiles only (K_e is symmetric): for G =
+// ceil(N / 4) node groups, tile t of an element covers rows 4I..4I+3 and columns 4J..4J+3, I <= J, in the
+// order (0,0), (0,1), .., (0,G-1), (1,1), ... Per quadrature point each lane (el, a) publishes node a's
+// physical gradient n_a and its weighted copy s_q n_a (adding node a's residual terms), and each tile lane
+// reads the weighted gradients of its 4 rows and the gradients of its 4 columns and adds the Gram sums of
+// its 16 blocks: 64 FMAs for 8 LDS reads.
+// Calls f(std::integral_constant<int, i>) for i = 0 .. n-1, each call a separate instantiation.
+template <typename F, int... Is>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, Is...>) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int n, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(f, std::make_integer_sequence<int, n>{});
+}
+
+template <int P>
+struct TileMap {
+  static constexpr int N = Ord<P>::N;
+  static constexpr int G = (N + 3) / 4;
+  static constexpr int T = G * (G + 1) / 2;
+  static_assert(Ord<P>::EPW * T <= 64, "the tiles of a wavefront's elements exceed its lanes");
+};
+
+template <int P, typename T>
+struct __align__(16) WaveLDSTile {
+  using O = Ord<P>;
+  using V2 = typename Vec2<T>::type;
+  static constexpr int PUB_STRIDE = odd_units(O::N, int(sizeof(V2)));
+  // Emission staging of one group of 4 rows: plane 0 holds (K_xx, K_xy) and plane 1 (K_yx, K_yy) of block
+  // (el, row i, column b) at kidx(el, i, b). Column b sits at b + b/4 and elements are an odd number of
+  // 2-vectors apart, so the lanes of one staging write (one per tile: columns 4 apart) hit distinct banks.
+  static constexpr int KROW = O::N + O::N / 4 + 1;
+  static constexpr int KEL = 4 * KROW + 1;
+  union {
+    double nodal[O::EPW][O::NODAL_STRIDE];  // x, y, u, v of each element node
+    V2 pub[2][2][O::EPW][PUB_STRIDE];       // [buffer][n_a, s_q n_a][el][a] at the current point
+    V2 kq[2][O::EPW * KEL];
+  };
+  double geo[O::EPW][O::GEO_STRIDE];  // as WaveLDS::geo
+
+  static __device__ __forceinline__ int kidx(int el, int i, int b) { return el * KEL + i * KROW + b + b / 4; }
+  __device__ __forceinline__ void put(int el, int i, int b, T k0, T k1, T k2, T k3) {
+    V2 lo, hi;
+    lo.x = k0;
+    lo.y = k1;
+    hi.x = k2;
+    hi.y = k3;
+    kq[0][kidx(el, i, b)] = lo;
+    kq[1][kidx(el, i, b)] = hi;
+  }
+  __device__ __forceinline__ T get(int el, int i, int b, int c) const {
+    return reinterpret_cast<const T *>(&kq[c >> 1][kidx(el, i, b)])[c & 1];
+  }
+};
+
+template <int P, typename T, int WAVES, bool RES>
+__global__ __launch_bounds__(WAVES * 64) void element_tiled_kernel(const ElemArgs args) {
+  using O = Ord<P>;
+  using TM = TileMap<P>;
+  using V2 = typename Vec2<T>::type;
+  constexpr int N = O::N, Q = O::Q, EPW = O::EPW, NT = TM::T;
+  static_assert(Q == N, "lane (el, q) of the geometry stage is lane (el, a)");
+  __shared__ WaveLDSTile<P, T> wl[WAVES];
+  __shared__ Basis1D basis;
+  __shared__ double2 gref[Q * N];
+  copy_basis(basis, args.basis);
+  fill_gref<P>(gref, args.basis);
+  __syncthreads();
+
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  WaveLDSTile<P, T> &W = wl[wave];
+  const int64_t first = (int64_t(blockIdx.x) * WAVES + wave) * EPW;  // the wavefront's first element slot
+  // node role: lane (el, a)
+  const int el = lane / N, a = lane % N;
+  const bool nvalid = lane < EPW * N && first + el < args.count;
+  const int elc = el < EPW ? el : EPW - 1;
+  int node = 0;
+  if (nvalid) {
+    node = args.conn[(first + el) * N + a];
+    store_nodal(W, el, a, *reinterpret_cast<const double2 *>(args.coords + 2 * int64_t(node)),
+                *reinterpret_cast<const double2 *>(args.states + 2 * int64_t(node)));
+  }
+  wave_sync();
+  if (nvalid) stage_geometry<P, T>(W, basis, args.mat, el, a);
+  wave_sync();
+  // tile role: lane (te, t)
+  const int te = lane / NT;
+  const bool tvalid = lane < EPW * NT && first + te < args.count;
+  const int tec = te < EPW ? te : EPW - 1;
+  int I = 0, J = 0;
+  {
+    int r = lane < EPW * NT ? lane % NT : 0;
+    while (r >= TM::G - I) {
+      r -= TM::G - I;
+      I++;
+    }
+    J = I + r;
+  }
+  int rows[4], cols[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    rows[k] = min(4 * I + k, N - 1);  // past the last node: a real node, not emitted
+    cols[k] = min(4 * J + k, N - 1);
+  }
+  T S[4][4][4];  // [row][col][xx, yy, xy, yx]
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) S[i][j][0] = S[i][j][1] = S[i][j][2] = S[i][j][3] = 0;
+  double Rx = 0, Ry = 0;
+#pragma unroll 1
+  for (int q = 0; q < Q; q++) {
+    const int buf = q & 1;
+    PointGeo G;
+    load_point<P, T>(W, gref, elc, a < N ? a : 0, q, G);
+    T nx, ny;
+    V2 v;
+    point_terms<T, V2>(G, nx, ny, v, Rx, Ry);
+    if (nvalid) {
+      W.pub[buf][0][el][a].x = nx;
+      W.pub[buf][0][el][a].y = ny;
+      W.pub[buf][1][el][a] = v;
+    }
+    wave_sync();  // a buffer is rewritten two points after it was read; a wavefront's LDS accesses run in order
+    V2 rw[4], cn[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      rw[k] = W.pub[buf][1][tec][rows[k]];
+      cn[k] = W.pub[buf][0][tec][cols[k]];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        S[i][j][0] = fma(rw[i].x, cn[j].x, S[i][j][0]);
+        S[i][j][1] = fma(rw[i].y, cn[j].y, S[i][j][1]);
+        S[i][j][2] = fma(rw[i].x, cn[j].y, S[i][j][2]);
+        S[i][j][3] = fma(rw[i].y, cn[j].x, S[i][j][3]);
+      }
+    }
+  }
+  if (RES && nvalid) {
+    const int rc = args.rcode[(first + el) * N + a];
+    double *r = rc < args.numNodes ? args.residual + 2 * int64_t(rc) : args.rstage + 2 * (int64_t(rc) - args.numNodes);
+    __builtin_nontemporal_store(Rx, r);
+    __builtin_nontemporal_store(Ry, r + 1);
+  }
+  // The tile's blocks K(a, b), row-major [xx, xy, yx, yy], in place of its Gram sums.
+  {
+    const T mu = T(args.mat.mu), lam = T(args.mat.lambda), c1 = lam + 2 * mu;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const T xx = S[i][j][0], yy = S[i][j][1], xy = S[i][j][2], yx = S[i][j][3];
+        S[i][j][0] = c1 * xx + mu * yy;
+        S[i][j][1] = lam * xy + mu * yx;
+        S[i][j][2] = lam * yx + mu * xy;
+        S[i][j][3] = c1 * yy + mu * xx;
+      }
+    }
+  }
+  // Emission, one group of 4 rows at a time: the tiles holding blocks of rows 4R..4R+3 write them to LDS
+  // (tile (R, J) directly; tile (J, R), J < R, transposed: K(b, a) = K(a, b)^T), then each store instruction
+  // covers the four components of 16 consecutive blocks of the group's rows (lanes 4k..4k+3: block k). The
+  // destination codes of group R+1 are loaded while group R is stored. Every code load and store is
+  // unconditional, so that the wait for a group's codes lets the previous group's stores stay in flight:
+  // lanes past the group's last block (or past the wavefront's last element) repeat the store of that block,
+  // the same value to the same address in the same instruction.
+  constexpr int ITER = (EPW * 4 * N + 15) / 16;  // store instructions of a full group of 4 rows
+  const int c = lane & 3;
+  const int lastEl = int(min<int64_t>(EPW, args.count - first)) - 1;
+  // block (e, rem) of store instruction `it` of group R (rem: offset within the element's rows of the group)
+  auto locate = [&](int R, int it, int &e, int &rem) __attribute__((always_inline)) {
+    const int groupBlocks = (N - 4 * R < 4 ? N - 4 * R : 4) * N;
+    rem = min(16 * it + (lane >> 2), EPW * groupBlocks - 1);
+    e = 0;
+#pragma unroll
+    for (int k = 1; k < EPW; k++) {
+      if (rem >= groupBlocks) {
+        rem -= groupBlocks;
+        e++;
+      }
+    }
+    if (e > lastEl) {
+      e = lastEl;
+      rem = groupBlocks - 1;
+    }
+  };
+  int cur[ITER], nxt[ITER];
+  auto load_codes = [&](auto Rc, int (&at)[ITER]) __attribute__((always_inline)) {
+    constexpr int R = decltype(Rc)::value;
+    constexpr int IT = (EPW * (N - 4 * R < 4 ? N - 4 * R : 4) * N + 15) / 16;
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      int e, rem;
+      locate(R, it, e, rem);
+      at[it] = args.map[((first + e) * N + 4 * R) * N + rem];
+    }
+  };
+  load_codes(std::integral_constant<int, 0>{}, cur);
+  static_for<TM::G>([&](auto Rc) __attribute__((always_inline)) {
+    constexpr int R = decltype(Rc)::value;
+    constexpr int IT = (EPW * (N - 4 * R < 4 ? N - 4 * R : 4) * N + 15) / 16;
+    if constexpr (R + 1 < TM::G) load_codes(std::integral_constant<int, R + 1>{}, nxt);
+    wave_sync();  // the previous group's (or the last point's) LDS reads are done
+    if (tvalid && I == R) {
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (4 * I + i < N && 4 * J + j < N) W.put(tec, i, 4 * J + j, S[i][j][0], S[i][j][1], S[i][j][2], S[i][j][3]);
+    } else if (tvalid && J == R) {
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (4 * J + j < N) W.put(tec, j, 4 * I + i, S[i][j][0], S[i][j][2], S[i][j][1], S[i][j][3]);
+    }
+    wave_sync();
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      int e, rem;
+      locate(R, it, e, rem);
+      const int at = cur[it];
+      const int i = rem / N, b = rem - i * N;
+      double *blkp = at < args.numBlocks ? args.values + 4 * int64_t(at) : args.stage + 4 * (int64_t(at) - args.numBlocks);
+      __builtin_nontemporal_store(double(W.get(e, i, b, c)), blkp + c);
+    }
+#pragma unroll
+    for (int it = 0; it < ITER; it++) cur[it] = nxt[it];
+  });
 }
 
 // ---- NONLINEAR (Green) strain: element-parallel drop-in paths ------------------------------------------
@@ -1921,10 +2157,10 @@ void launch_element_order(int order, const ElemArgs &a, int precision, hipStream
 
 template <int P, typename T, bool RES>
 void launch_element_split(const ElemArgs &a, hipStream_t stream) {
-  constexpr int W = elem_waves(P);
+  constexpr int W = 2;
   constexpr int per = W * Ord<P>::EPW;
   const int grid = (a.count + per - 1) / per;
-  if (grid > 0) element_kernel<P, T, W, RES, true, false, true><<<grid, W * 64, 0, stream>>>(a);
+  if (grid > 0) element_tiled_kernel<P, T, W, RES><<<grid, W * 64, 0, stream>>>(a);
 }
 
 // Second pass of the element-once assembly: output dst[i] (C doubles from C * dst[i]) is the sum of the
