@@ -35,6 +35,8 @@ WORKLOADS = {
                                "1250 circumferential order-4 quads, 5x5 Gauss, smooth field + U(+-1e-5)"),
     "square64-p1": dict(kind="square", n=64, order=1, jitter=0.0, desc="unit square 64x64 order-1 quads, 2x2 Gauss, "
                         "u,v ~ U(+-1e-3) seed 1"),
+    "square500-p3": dict(kind="square", n=500, order=3, jitter=0.2, desc="unit square 500x500 order-3 quads, "
+                         "4x4 Gauss, interior jitter U(+-0.2h) seed 2, u,v ~ U(+-1e-3) seed 3"),
 }
 POOL_WORKLOADS = {
     "batch256-mixed": "256 independent meshes (parameter seed 6, mesh i seed 6e6+i): order U{1..4}, elements "
@@ -69,7 +71,7 @@ def plan_kernel_name(args, order):
     element-once traversal that stores exclusive blocks and stages shared ones, then the gathers of the
     staged sums."""
     if order >= 3:
-        return "element_kernel+gather_shared"
+        return "element_tiled_kernel+gather_shared"
     if args.path == "plan-det":
         return "owner_kernel"
     return "owner_stream_kernel"

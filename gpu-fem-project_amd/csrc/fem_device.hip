@@ -415,18 +415,9 @@ struct ElemArgs {
   double *values;
   Basis1D basis;
   Material mat;
-  // Element-once assembly of a resident plan (SPLIT kernels): conn is [E][N] (connPtr unused) and map holds
-  // [E][N][N] block codes, row-major: a code below numBlocks is the BSR index of a block this element alone
-  // forms (stored directly), else numBlocks + the block's slot in stage (summed by gather_shared). rcode
-  // [E][N] codes each node's residual likewise, against numNodes and rstage. Every store is plain.
-  double *stage;
-  const int *rcode;
-  double *rstage;
-  int numBlocks;
-  int numNodes;
 };
 
-template <int P, typename T, int WAVES, bool RES, bool JAC, bool ATOMIC, bool SPLIT = false>
+template <int P, typename T, int WAVES, bool RES, bool JAC, bool ATOMIC>
 __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args) {
   using O = Ord<P>;
   constexpr int N = O::N;
@@ -447,7 +438,7 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   int eid = 0, node = 0;
   if (valid) {
     eid = args.elemList ? args.elemList[slot] : args.first + int(slot);
-    node = SPLIT ? args.conn[int64_t(eid) * N + idx] : args.conn[args.connPtr[eid] + idx];
+    node = args.conn[args.connPtr[eid] + idx];
     store_nodal(W, el, idx, *reinterpret_cast<const double2 *>(args.coords + 2 * int64_t(node)),
                 *reinterpret_cast<const double2 *>(args.states + 2 * int64_t(node)));
   }
@@ -484,15 +475,9 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
         const int src = (lane >> 2) + 16 * r;
         const int a = stAt[wave][src];
         if (a >= 0) {
-          if constexpr (SPLIT) {
-            double *blk = a < args.numBlocks ? args.values + 4 * int64_t(a)
-                                             : args.stage + 4 * (int64_t(a) - args.numBlocks);
-            __builtin_nontemporal_store(stK[wave][c][src], blk + c);
-          } else {
-            double *blk = args.values + a + c;
-            if (ATOMIC) unsafeAtomicAdd(blk, stK[wave][c][src]);
-            else *blk += stK[wave][c][src];
-          }
+          double *blk = args.values + a + c;
+          if (ATOMIC) unsafeAtomicAdd(blk, stK[wave][c][src]);
+          else *blk += stK[wave][c][src];
         }
       }
       wave_sync();  // the next row restages into the same slots
@@ -503,11 +488,7 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   }
   if (RES && valid) {
     double *r = args.residual + 2 * int64_t(node);
-    if constexpr (SPLIT) {
-      const int rc = args.rcode[int64_t(eid) * N + idx];
-      if (rc >= args.numNodes) r = args.rstage + 2 * (int64_t(rc) - args.numNodes);
-      *reinterpret_cast<double2 *>(r) = make_double2(Rx, Ry);
-    } else if (ATOMIC) {
+    if (ATOMIC) {
       unsafeAtomicAdd(r, double(Rx));
       unsafeAtomicAdd(r + 1, double(Ry));
     } else {
@@ -574,39 +555,48 @@ struct __align__(16) WaveLDSTile {
   }
 };
 
-template <int P, typename T, int WAVES, bool RES>
-__global__ __launch_bounds__(WAVES * 64) void element_tiled_kernel(const ElemArgs args) {
+// One element-once assembly: a plan's element-once tables and a request's inputs and outputs.
+struct SplitJob {
+  const int *conn;   // [E][N]
+  const int *code;   // [E][N][N] block codes (SplitHost::code)
+  const int *rcode;  // [E][N] residual codes
+  const int *gdst, *gptr, *rdst, *rptr;  // gather lists of the staged blocks and residual entries
+  const double *states, *coords;
+  double *residual;  // NULL: no residual
+  double *values;
+  double *stage, *rstage;
+  Material mat;
+  int count, numBlocks, numNodes, numGather, numRgather;
+};
+
+// Elements first .. first+EPW-1 of `job` (those below job.count) on one wavefront: geometry, Gram tiles,
+// residual and emission. Every lane of the wavefront calls it; it synchronises the wavefront only.
+template <int P, typename T, bool RES>
+__device__ __forceinline__ void tiled_wave(WaveLDSTile<P, T> &W, const Basis1D &basis, const double2 *gref,
+                                           const SplitJob &job, int64_t first, int lane) {
   using O = Ord<P>;
   using TM = TileMap<P>;
   using V2 = typename Vec2<T>::type;
   constexpr int N = O::N, Q = O::Q, EPW = O::EPW, NT = TM::T;
   static_assert(Q == N, "lane (el, q) of the geometry stage is lane (el, a)");
-  __shared__ WaveLDSTile<P, T> wl[WAVES];
-  __shared__ Basis1D basis;
-  __shared__ double2 gref[Q * N];
-  copy_basis(basis, args.basis);
-  fill_gref<P>(gref, args.basis);
-  __syncthreads();
-
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  WaveLDSTile<P, T> &W = wl[wave];
-  const int64_t first = (int64_t(blockIdx.x) * WAVES + wave) * EPW;  // the wavefront's first element slot
+  wave_sync();  // a previous call's LDS reads are done before the nodes overwrite them
+  if (first >= job.count) return;  // no element: the emission below assumes at least one
   // node role: lane (el, a)
   const int el = lane / N, a = lane % N;
-  const bool nvalid = lane < EPW * N && first + el < args.count;
+  const bool nvalid = lane < EPW * N && first + el < job.count;
   const int elc = el < EPW ? el : EPW - 1;
   int node = 0;
   if (nvalid) {
-    node = args.conn[(first + el) * N + a];
-    store_nodal(W, el, a, *reinterpret_cast<const double2 *>(args.coords + 2 * int64_t(node)),
-                *reinterpret_cast<const double2 *>(args.states + 2 * int64_t(node)));
+    node = job.conn[(first + el) * N + a];
+    store_nodal(W, el, a, *reinterpret_cast<const double2 *>(job.coords + 2 * int64_t(node)),
+                *reinterpret_cast<const double2 *>(job.states + 2 * int64_t(node)));
   }
   wave_sync();
-  if (nvalid) stage_geometry<P, T>(W, basis, args.mat, el, a);
+  if (nvalid) stage_geometry<P, T>(W, basis, job.mat, el, a);
   wave_sync();
   // tile role: lane (te, t)
   const int te = lane / NT;
-  const bool tvalid = lane < EPW * NT && first + te < args.count;
+  const bool tvalid = lane < EPW * NT && first + te < job.count;
   const int tec = te < EPW ? te : EPW - 1;
   int I = 0, J = 0;
   {
@@ -661,14 +651,14 @@ __global__ __launch_bounds__(WAVES * 64) void element_tiled_kernel(const ElemArg
     }
   }
   if (RES && nvalid) {
-    const int rc = args.rcode[(first + el) * N + a];
-    double *r = rc < args.numNodes ? args.residual + 2 * int64_t(rc) : args.rstage + 2 * (int64_t(rc) - args.numNodes);
+    const int rc = job.rcode[(first + el) * N + a];
+    double *r = rc < job.numNodes ? job.residual + 2 * int64_t(rc) : job.rstage + 2 * (int64_t(rc) - job.numNodes);
     __builtin_nontemporal_store(Rx, r);
     __builtin_nontemporal_store(Ry, r + 1);
   }
   // The tile's blocks K(a, b), row-major [xx, xy, yx, yy], in place of its Gram sums.
   {
-    const T mu = T(args.mat.mu), lam = T(args.mat.lambda), c1 = lam + 2 * mu;
+    const T mu = T(job.mat.mu), lam = T(job.mat.lambda), c1 = lam + 2 * mu;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
 #pragma unroll
@@ -690,7 +680,7 @@ __global__ __launch_bounds__(WAVES * 64) void element_tiled_kernel(const ElemArg
   // the same value to the same address in the same instruction.
   constexpr int ITER = (EPW * 4 * N + 15) / 16;  // store instructions of a full group of 4 rows
   const int c = lane & 3;
-  const int lastEl = int(min<int64_t>(EPW, args.count - first)) - 1;
+  const int lastEl = int(min<int64_t>(EPW, job.count - first)) - 1;
   // block (e, rem) of store instruction `it` of group R (rem: offset within the element's rows of the group)
   auto locate = [&](int R, int it, int &e, int &rem) __attribute__((always_inline)) {
     const int groupBlocks = (N - 4 * R < 4 ? N - 4 * R : 4) * N;
@@ -716,7 +706,7 @@ __global__ __launch_bounds__(WAVES * 64) void element_tiled_kernel(const ElemArg
     for (int it = 0; it < IT; it++) {
       int e, rem;
       locate(R, it, e, rem);
-      at[it] = args.map[((first + e) * N + 4 * R) * N + rem];
+      at[it] = job.code[((first + e) * N + 4 * R) * N + rem];
     }
   };
   load_codes(std::integral_constant<int, 0>{}, cur);
@@ -745,12 +735,62 @@ __global__ __launch_bounds__(WAVES * 64) void element_tiled_kernel(const ElemArg
       locate(R, it, e, rem);
       const int at = cur[it];
       const int i = rem / N, b = rem - i * N;
-      double *blkp = at < args.numBlocks ? args.values + 4 * int64_t(at) : args.stage + 4 * (int64_t(at) - args.numBlocks);
+      double *blkp = at < job.numBlocks ? job.values + 4 * int64_t(at) : job.stage + 4 * (int64_t(at) - job.numBlocks);
       __builtin_nontemporal_store(double(W.get(e, i, b, c)), blkp + c);
     }
 #pragma unroll
     for (int it = 0; it < ITER; it++) cur[it] = nxt[it];
   });
+}
+
+
+template <int P, typename T, int WAVES, bool RES>
+__global__ __launch_bounds__(WAVES * 64) void element_tiled_kernel(const SplitJob job, const Basis1D basisIn) {
+  using O = Ord<P>;
+  __shared__ WaveLDSTile<P, T> wl[WAVES];
+  __shared__ Basis1D basis;
+  __shared__ double2 gref[O::Q * O::N];
+  copy_basis(basis, basisIn);
+  fill_gref<P>(gref, basisIn);
+  __syncthreads();
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  tiled_wave<P, T, RES>(wl[wave], basis, gref, job, (int64_t(blockIdx.x) * WAVES + wave) * O::EPW, lane);
+}
+
+// Index j of the last start[j] <= x over start[0..n) (start ascending, start[0] = 0 <= x).
+__device__ __forceinline__ int find_job(const int *start, int n, int64_t x) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (start[mid] <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// A batch's element-once jobs of one order: counts = (jobs, units, staged blocks, staged residual entries)
+// in device memory (so a recorded graph replays any batch), unitStart[j] the first unit of job j, a unit
+// being WAVES * EPW elements. Persistent: workgroups stride over the units.
+template <int P, typename T, int WAVES, bool RES>
+__global__ __launch_bounds__(WAVES * 64) void element_tiled_batch(const SplitJob *__restrict__ jobs,
+                                                                   const int *__restrict__ unitStart,
+                                                                   const int *__restrict__ counts,
+                                                                   const Basis1D basisIn) {
+  using O = Ord<P>;
+  __shared__ WaveLDSTile<P, T> wl[WAVES];
+  __shared__ Basis1D basis;
+  __shared__ double2 gref[O::Q * O::N];
+  copy_basis(basis, basisIn);
+  fill_gref<P>(gref, basisIn);
+  __syncthreads();
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int numJobs = counts[0], units = counts[1];
+  for (int u = blockIdx.x; u < units; u += gridDim.x) {
+    const int j = __builtin_amdgcn_readfirstlane(find_job(unitStart, numJobs, u));  // uniform: scalar loads
+    const SplitJob job = jobs[j];
+    const int64_t first = (int64_t(u - unitStart[j]) * WAVES + wave) * O::EPW;
+    tiled_wave<P, T, RES>(wl[wave], basis, gref, job, first, lane);
+  }
 }
 
 // ---- NONLINEAR (Green) strain: element-parallel drop-in paths ------------------------------------------
@@ -2155,28 +2195,33 @@ void launch_element_order(int order, const ElemArgs &a, int precision, hipStream
   }
 }
 
+constexpr int kTiledWaves = 2;  // wavefronts per workgroup of the element-once kernels
+
 template <int P, typename T, bool RES>
-void launch_element_split(const ElemArgs &a, hipStream_t stream) {
-  constexpr int W = 2;
-  constexpr int per = W * Ord<P>::EPW;
-  const int grid = (a.count + per - 1) / per;
-  if (grid > 0) element_tiled_kernel<P, T, W, RES><<<grid, W * 64, 0, stream>>>(a);
+void launch_element_split(const SplitJob &job, const Basis1D &basis, hipStream_t stream) {
+  constexpr int per = kTiledWaves * Ord<P>::EPW;
+  const int grid = (job.count + per - 1) / per;
+  if (grid > 0) element_tiled_kernel<P, T, kTiledWaves, RES><<<grid, kTiledWaves * 64, 0, stream>>>(job, basis);
 }
 
 // Second pass of the element-once assembly: output dst[i] (C doubles from C * dst[i]) is the sum of the
 // staged contributions in slots ptr[i] .. ptr[i+1]-1, in slot order; zero if there are none. Lane t covers
 // component t % C of output t / C, so a wavefront reads consecutive slots.
 template <int C>
+__device__ __forceinline__ void gather_one(const int *dst, const int *ptr, const double *stage, double *out, int64_t i,
+                                           int c) {
+  const int k0 = ptr[i], k1 = ptr[i + 1];
+  double sum = 0;
+  for (int k = k0; k < k1; k++) sum += __builtin_nontemporal_load(stage + int64_t(C) * k + c);
+  __builtin_nontemporal_store(sum, out + int64_t(C) * dst[i] + c);
+}
+
+template <int C>
 __global__ __launch_bounds__(256) void gather_shared(const int *dst, const int *ptr, const double *stage,
                                                      double *out, int n) {
   const int64_t total = int64_t(n) * C, stride = int64_t(gridDim.x) * 256;
   for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total; t += stride) {
-    const int64_t i = t / C;
-    const int c = int(t % C);
-    const int k0 = ptr[i], k1 = ptr[i + 1];
-    double sum = 0;
-    for (int k = k0; k < k1; k++) sum += __builtin_nontemporal_load(stage + int64_t(C) * k + c);
-    __builtin_nontemporal_store(sum, out + int64_t(C) * dst[i] + c);
+    gather_one<C>(dst, ptr, stage, out, t / C, int(t % C));
   }
 }
 
@@ -2187,20 +2232,36 @@ void launch_gather_shared(const int *dst, const int *ptr, const double *stage, d
   if (n > 0) gather_shared<C><<<unsigned(std::min<int64_t>(blocks, 1 << 20)), 256, 0, stream>>>(dst, ptr, stage, out, n);
 }
 
-// Element-once assembly of a plan with split tables: the element kernel stores exclusive outputs and stages
-// the rest, then the gathers store the staged sums.
-template <int P>
-void launch_split(const fem_plan *plan, ElemArgs a, int precision, hipStream_t s) {
-  const bool res = a.residual != nullptr;
-  if (precision == FEM_PREC_MIXED) {
-    if (res) launch_element_split<P, float, true>(a, s);
-    else launch_element_split<P, float, false>(a, s);
-  } else {
-    if (res) launch_element_split<P, double, true>(a, s);
-    else launch_element_split<P, double, false>(a, s);
+// The gathers of a batch's element-once jobs: C = 4, the staged blocks (outStart[j]: job j's first output
+// over all jobs, counts[2] in total), or C = 2, the staged residual entries (counts[3]; jobs without a
+// residual have none).
+template <int C>
+__global__ __launch_bounds__(256) void gather_batch(const SplitJob *jobs, const int *outStart, const int *counts) {
+  const int numJobs = counts[0];
+  const int64_t total = int64_t(counts[C == 4 ? 2 : 3]) * C, stride = int64_t(gridDim.x) * 256;
+  for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total; t += stride) {
+    const int64_t i = t / C;
+    const int j = find_job(outStart, numJobs, i);
+    const SplitJob &job = jobs[j];
+    if (C == 4) gather_one<4>(job.gdst, job.gptr, job.stage, job.values, i - outStart[j], int(t % C));
+    else gather_one<2>(job.rdst, job.rptr, job.rstage, job.residual, i - outStart[j], int(t % C));
   }
-  launch_gather_shared<4>(plan->dGdst, plan->dGptr, plan->dStage, a.values, plan->numGather, s);
-  if (res) launch_gather_shared<2>(plan->dRdst, plan->dRptr, plan->dRstage, a.residual, plan->numRgather, s);
+}
+
+// Element-once assembly of one plan: the element kernel stores exclusive outputs and stages the rest, then
+// the gathers store the staged sums.
+template <int P>
+void launch_split(const SplitJob &job, const Basis1D &basis, int precision, hipStream_t s) {
+  const bool res = job.residual != nullptr;
+  if (precision == FEM_PREC_MIXED) {
+    if (res) launch_element_split<P, float, true>(job, basis, s);
+    else launch_element_split<P, float, false>(job, basis, s);
+  } else {
+    if (res) launch_element_split<P, double, true>(job, basis, s);
+    else launch_element_split<P, double, false>(job, basis, s);
+  }
+  launch_gather_shared<4>(job.gdst, job.gptr, job.stage, job.values, job.numGather, s);
+  if (res) launch_gather_shared<2>(job.rdst, job.rptr, job.rstage, job.residual, job.numRgather, s);
 }
 
 template <int P, bool RES, bool JAC>
@@ -2310,6 +2371,31 @@ bool plan_streams(const fem_plan *p) {
 }
 
 int stream_grid(int numCUs) { return ((2 * numCUs + 7) / 8) * 8; }
+
+SplitJob make_split_job(const fem_plan *p, const double *states, const double *coords, double *residual,
+                        double *values, const Material &m) {
+  SplitJob j{};
+  j.conn = p->dConn;
+  j.code = p->dCode;
+  j.rcode = p->dRcode;
+  j.gdst = p->dGdst;
+  j.gptr = p->dGptr;
+  j.rdst = p->dRdst;
+  j.rptr = p->dRptr;
+  j.states = states;
+  j.coords = coords;
+  j.residual = residual;
+  j.values = values;
+  j.stage = p->dStage;
+  j.rstage = p->dRstage;
+  j.mat = m;
+  j.count = p->numElements;
+  j.numBlocks = int(p->numBlocks);
+  j.numNodes = p->numNodes;
+  j.numGather = p->numGather;
+  j.numRgather = residual ? p->numRgather : 0;
+  return j;
+}
 
 MeshJob make_job(const fem_plan *p, const double *states, const double *coords, double *residual, double *values,
                  const Material &m) {
@@ -2725,24 +2811,10 @@ int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCo
   if (plan->dConn && matEntries) {
     // Element-once assembly (orders 3-4): exclusive outputs stored by their element, the rest staged and
     // gathered. Deterministic, so it also serves FEM_PLAN_DETERMINISTIC plans.
-    ElemArgs a{};
-    a.conn = plan->dConn;
-    a.count = plan->numElements;
-    a.states = states;
-    a.coords = nodeCoords;
-    a.map = plan->dCode;
-    a.residual = residual;
-    a.values = matEntries;
-    a.stage = plan->dStage;
-    a.rcode = plan->dRcode;
-    a.rstage = plan->dRstage;
-    a.numBlocks = int(plan->numBlocks);
-    a.numNodes = plan->numNodes;
-    a.basis = L.args.basis;
-    a.mat = m;
+    const SplitJob job = make_split_job(plan, states, nodeCoords, residual, matEntries, m);
     return timed(s, elapsedSeconds, [&]() {
-      if (plan->order == 3) launch_split<3>(plan, a, precision, s);
-      else launch_split<4>(plan, a, precision, s);
+      if (plan->order == 3) launch_split<3>(job, L.args.basis, precision, s);
+      else launch_split<4>(job, L.args.basis, precision, s);
       return FEM_OK;
     });
   }
@@ -2762,25 +2834,38 @@ int fem_plan_destroy(fem_plan *plan) {
 // ---- batches of resident plans ------------------------------------------------------------------
 }  // extern "C"
 
-// 12 launch groups: (order - 1) * 3 + outputs, outputs 0: residual, 1: values, 2: both.
+// 12 owner launch groups: (order - 1) * 3 + outputs, outputs 0: residual, 1: values, 2: both; and 4
+// element-once groups (Jacobian jobs of orders 3-4 whose plans have element-once tables): (order - 3) * 2 +
+// (1 if with residual).
 struct fem_batch {
   struct Group {
     std::vector<MeshJob> jobs;
     std::vector<int> jobStart{0};
     bool stream = true;
   };
+  struct SplitGroup {
+    std::vector<SplitJob> jobs;
+    std::vector<int> unitStart{0}, gStart{0}, rStart{0};  // prefix sums: units, staged blocks, staged residuals
+  };
   static constexpr int kGroups = 12;
+  static constexpr int kSplitGroups = 4;
   Group groups[kGroups];
+  SplitGroup split[kSplitGroups];
+  std::vector<const fem_plan *> splitPlans;  // plans with a job in `split` (their staging buffers are in use)
   int maxJobs = 0;
   int flags = 0;
   int numCUs = 0;
   int numJobs = 0;
   // One device block at a fixed address (a captured graph refers to it), laid out as
-  //   counts [kGroups][4] int (numJobs, totalParts) | jobStart [kGroups][maxJobs + 1] int | jobs [kGroups][maxJobs]
+  //   counts [kGroups + kSplitGroups][4] int (owner: numJobs, totalParts; element-once: numJobs, units,
+  //     staged blocks, staged residuals)
+  //   | jobStart [kGroups][maxJobs + 1] int | jobs [kGroups][maxJobs] MeshJob
+  //   | split starts [kSplitGroups][3][maxJobs + 1] int (unitStart, gStart, rStart)
+  //   | split jobs [kSplitGroups][maxJobs] SplitJob
   // and mirrored in pinned host memory, so each assembly stages its tables with one asynchronous copy.
   unsigned char *dTables = nullptr;
   unsigned char *hTables = nullptr;
-  size_t jobStartOffset = 0, jobsOffset = 0, tableBytes = 0;
+  size_t jobStartOffset = 0, jobsOffset = 0, splitStartOffset = 0, splitJobsOffset = 0, tableBytes = 0;
   hipEvent_t staged = nullptr;  // recorded after the copy: the host mirror is rewritten only once it has run
   bool stagedPending = false;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // per precision
@@ -2792,6 +2877,9 @@ struct fem_batch {
   int *hCounts() const { return reinterpret_cast<int *>(hTables); }
   int *hJobStart() const { return reinterpret_cast<int *>(hTables + jobStartOffset); }
   MeshJob *hJobs() const { return reinterpret_cast<MeshJob *>(hTables + jobsOffset); }
+  // element-once group g: start array k (0 units, 1 staged blocks, 2 staged residuals) and jobs
+  size_t splitStartAt(int g, int k) const { return splitStartOffset + (size_t(g) * 3 + k) * (maxJobs + 1) * sizeof(int); }
+  size_t splitJobsAt(int g) const { return splitJobsOffset + size_t(g) * maxJobs * sizeof(SplitJob); }
 };
 
 namespace {
@@ -2829,6 +2917,38 @@ OwnerLaunch batch_group_launch(const fem_batch *b, int gi, int precision, bool f
   return L;
 }
 
+// Launches of element-once group g (order 3 + g / 2; with residual if g is odd): the tiled kernel over the
+// group's units, then the gathers of its staged blocks and residual entries. The kernels read the group's
+// counts from device memory; `fixedGrid` sizes the grids for any batch (graph recording), otherwise from the
+// host's counts.
+template <int P, typename T, bool RES>
+void launch_split_group_t(const fem_batch *b, int g, bool fixedGrid, hipStream_t s) {
+  const fem_batch::SplitGroup &sg = b->split[g];
+  const int *counts = b->dCounts() + 4 * (fem_batch::kGroups + g);
+  const SplitJob *jobs = reinterpret_cast<const SplitJob *>(b->dTables + b->splitJobsAt(g));
+  const int *unitStart = reinterpret_cast<const int *>(b->dTables + b->splitStartAt(g, 0));
+  const int *gStart = reinterpret_cast<const int *>(b->dTables + b->splitStartAt(g, 1));
+  const int *rStart = reinterpret_cast<const int *>(b->dTables + b->splitStartAt(g, 2));
+  const int cap = std::max(8 * b->numCUs, 64);
+  auto grid = [&](int64_t work) { return unsigned(fixedGrid ? cap : std::max<int64_t>(1, std::min<int64_t>(work, cap))); };
+  Basis1D basis;
+  make_basis(P, basis);
+  element_tiled_batch<P, T, kTiledWaves, RES><<<grid(sg.unitStart.back()), kTiledWaves * 64, 0, s>>>(jobs, unitStart,
+                                                                                                   counts, basis);
+  gather_batch<4><<<grid((int64_t(sg.gStart.back()) * 4 + 255) / 256), 256, 0, s>>>(jobs, gStart, counts);
+  if (RES) gather_batch<2><<<grid((int64_t(sg.rStart.back()) * 2 + 255) / 256), 256, 0, s>>>(jobs, rStart, counts);
+}
+
+void launch_split_group(const fem_batch *b, int g, int precision, bool fixedGrid, hipStream_t s) {
+  const bool mixed = precision == FEM_PREC_MIXED;
+  switch (g) {
+    case 0: mixed ? launch_split_group_t<3, float, false>(b, g, fixedGrid, s) : launch_split_group_t<3, double, false>(b, g, fixedGrid, s); break;
+    case 1: mixed ? launch_split_group_t<3, float, true>(b, g, fixedGrid, s) : launch_split_group_t<3, double, true>(b, g, fixedGrid, s); break;
+    case 2: mixed ? launch_split_group_t<4, float, false>(b, g, fixedGrid, s) : launch_split_group_t<4, double, false>(b, g, fixedGrid, s); break;
+    default: mixed ? launch_split_group_t<4, float, true>(b, g, fixedGrid, s) : launch_split_group_t<4, double, true>(b, g, fixedGrid, s); break;
+  }
+}
+
 // Graph of the 12 group launches, each reading its job count from device memory (empty groups exit).
 // Recorded on the batch's own stream (capture is not allowed on the null stream); it is replayed on the
 // caller's stream.
@@ -2842,6 +2962,7 @@ int capture_batch_graph(fem_batch *b, int precision) {
     L.args.totalParts = L.streamGrid;  // host-side grid bound only; the kernel reads the device count
     launch_owner_any(gi / 3 + 1, L, s);
   }
+  for (int g = 0; g < fem_batch::kSplitGroups; g++) launch_split_group(b, g, precision, true, s);
   FEM_HIP_CHECK(hipStreamEndCapture(s, &graph));
   hipError_t e = hipGraphInstantiate(&b->graph[precision], graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
@@ -2862,10 +2983,12 @@ int fem_batch_create(int maxJobs, int flags, fem_batch **batch) {
   int dev = 0;
   (void)hipGetDevice(&dev);
   if (hipDeviceGetAttribute(&b->numCUs, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) b->numCUs = 0;
-  const size_t G = fem_batch::kGroups;
-  b->jobStartOffset = 4 * G * sizeof(int);
+  const size_t G = fem_batch::kGroups, SG = fem_batch::kSplitGroups;
+  b->jobStartOffset = 4 * (G + SG) * sizeof(int);
   b->jobsOffset = (b->jobStartOffset + G * size_t(maxJobs + 1) * sizeof(int) + 255) / 256 * 256;
-  b->tableBytes = b->jobsOffset + G * size_t(maxJobs) * sizeof(MeshJob);
+  b->splitStartOffset = b->jobsOffset + G * size_t(maxJobs) * sizeof(MeshJob);
+  b->splitJobsOffset = (b->splitStartOffset + SG * 3 * size_t(maxJobs + 1) * sizeof(int) + 255) / 256 * 256;
+  b->tableBytes = b->splitJobsOffset + SG * size_t(maxJobs) * sizeof(SplitJob);
   if (hipMalloc(reinterpret_cast<void **>(&b->dTables), b->tableBytes) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void **>(&b->hTables), b->tableBytes, hipHostMallocDefault) != hipSuccess ||
       hipEventCreateWithFlags(&b->staged, hipEventDisableTiming) != hipSuccess) {
@@ -2890,6 +3013,13 @@ int fem_batch_clear(fem_batch *batch) {
     g.jobStart.assign(1, 0);
     g.stream = true;
   }
+  for (auto &g : batch->split) {
+    g.jobs.clear();
+    g.unitStart.assign(1, 0);
+    g.gStart.assign(1, 0);
+    g.rStart.assign(1, 0);
+  }
+  batch->splitPlans.clear();
   batch->numJobs = 0;
   return FEM_OK;
 }
@@ -2902,12 +3032,36 @@ int fem_batch_add(fem_batch *batch, fem_plan *plan, const double *states, const 
   if ((plan->flags & FEM_PLAN_DETERMINISTIC) || (plan->flags & FEM_PLAN_RESIDUAL_ONLY)) {
     return set_error(FEM_ERR_UNSUPPORTED, "batches take plans built with default flags");
   }
-  if ((batch->flags & FEM_BATCH_GRAPH) && !plan_streams(plan)) {
-    return set_error(FEM_ERR_UNSUPPORTED, "graph batches need plans whose parts are one batch per wavefront");
-  }
   Material m;
   int rc = make_material(E, nu, t, m);
   if (rc) return rc;
+  // Jacobian jobs of plans with element-once tables take the element-once groups, one job per plan and
+  // batch (the plan's staging buffer); a further job on the same plan takes the owner path.
+  if (plan->dConn && matEntries &&
+      std::find(batch->splitPlans.begin(), batch->splitPlans.end(), plan) == batch->splitPlans.end()) {
+    fem_batch::SplitGroup &g = batch->split[(plan->order - 3) * 2 + (residual ? 1 : 0)];
+    if (int(g.jobs.size()) >= batch->maxJobs) {
+      return set_error(FEM_ERR_INVALID_ARGUMENT, "batch group is full (%d jobs)", batch->maxJobs);
+    }
+    const SplitJob job = make_split_job(plan, states, nodeCoords, residual, matEntries, m);
+    const int per = kTiledWaves * (plan->order == 3 ? Ord<3>::EPW : Ord<4>::EPW);
+    const int64_t units = g.unitStart.back() + (int64_t(plan->numElements) + per - 1) / per;
+    const int64_t staged = int64_t(g.gStart.back()) + job.numGather;
+    const int64_t rstaged = int64_t(g.rStart.back()) + job.numRgather;
+    if (units > INT32_MAX || staged > INT32_MAX || rstaged > INT32_MAX) {
+      return set_error(FEM_ERR_UNSUPPORTED, "batch group exceeds int work counts");
+    }
+    g.jobs.push_back(job);
+    g.unitStart.push_back(int(units));
+    g.gStart.push_back(int(staged));
+    g.rStart.push_back(int(rstaged));
+    batch->splitPlans.push_back(plan);
+    batch->numJobs++;
+    return FEM_OK;
+  }
+  if ((batch->flags & FEM_BATCH_GRAPH) && !plan_streams(plan)) {
+    return set_error(FEM_ERR_UNSUPPORTED, "graph batches need plans whose parts are one batch per wavefront");
+  }
   const int outputs = (residual && matEntries) ? 2 : (matEntries ? 1 : 0);
   fem_batch::Group &g = batch->groups[(plan->order - 1) * 3 + outputs];
   if (int(g.jobs.size()) >= batch->maxJobs) {
@@ -2945,6 +3099,21 @@ int fem_batch_assemble(fem_batch *batch, int precision, void *stream, double *el
                 g.jobStart.size() * sizeof(int));
     copyEnd = batch->jobsOffset + (size_t(gi) * batch->maxJobs + g.jobs.size()) * sizeof(MeshJob);
   }
+  for (int g = 0; g < fem_batch::kSplitGroups; g++) {
+    const fem_batch::SplitGroup &sg = batch->split[g];
+    int *cnt = batch->hCounts() + 4 * (fem_batch::kGroups + g);
+    cnt[0] = int(sg.jobs.size());
+    cnt[1] = sg.unitStart.back();
+    cnt[2] = sg.gStart.back();
+    cnt[3] = sg.rStart.back();
+    if (sg.jobs.empty()) continue;
+    const std::vector<int> *starts[3] = {&sg.unitStart, &sg.gStart, &sg.rStart};
+    for (int k = 0; k < 3; k++) {
+      std::memcpy(batch->hTables + batch->splitStartAt(g, k), starts[k]->data(), starts[k]->size() * sizeof(int));
+    }
+    std::memcpy(batch->hTables + batch->splitJobsAt(g), sg.jobs.data(), sg.jobs.size() * sizeof(SplitJob));
+    copyEnd = batch->splitJobsAt(g) + sg.jobs.size() * sizeof(SplitJob);
+  }
   FEM_HIP_CHECK(hipMemcpyAsync(batch->dTables, batch->hTables, copyEnd, hipMemcpyHostToDevice, s));
   FEM_HIP_CHECK(hipEventRecord(batch->staged, s));
   batch->stagedPending = true;
@@ -2974,6 +3143,9 @@ int fem_batch_assemble(fem_batch *batch, int precision, void *stream, double *el
           launch_owner_any(gi / 3 + 1, Lj, s);
         }
       }
+    }
+    for (int g = 0; g < fem_batch::kSplitGroups; g++) {
+      if (!batch->split[g].jobs.empty()) launch_split_group(batch, g, precision, false, s);
     }
     return FEM_OK;
   });

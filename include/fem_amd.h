@@ -167,7 +167,12 @@ int fem_jacobian_coloured(int order, const int *connPtr, const int *conn, int nu
  * rows fit a workgroup's LDS; each workgroup assembles every element touching its rows and writes
  * its residual entries and BSR blocks exactly once (no atomics to HBM, no zero-fill). Results are
  * written (not accumulated) into the caller's residual / values in the layout of `rowp`/`cols`.
- * nodeCoordsHost (optional) orders nodes along a Morton curve so parts are compact. */
+ * nodeCoordsHost (optional) orders nodes along a Morton curve so parts are compact.
+ * At orders 3-4 a plan also holds element-once tables: a Jacobian assembly visits each element once,
+ * stores the blocks (and residual entries) that element alone forms, stages the others in the plan's
+ * staging buffer and sums them in a gather pass; every output is still written exactly once, in a fixed
+ * order, so results are bitwise reproducible with or without FEM_PLAN_DETERMINISTIC. A plan's job record
+ * and staging buffer are per plan: assemblies of one plan must not overlap on different streams. */
 typedef struct fem_plan fem_plan;
 
 int fem_plan_create(int order, int numNodes, int numElements, const int *connPtr, const int *conn,

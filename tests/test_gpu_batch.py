@@ -93,3 +93,81 @@ def test_batch_rejects_deterministic_plans(fem, torch_gpu, meshes):
         batch.add(plan, t, t, E, NU, T, res=t)
     batch.close()
     plan.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_element_once_groups(fem, torch_gpu, meshes, graph):
+    """Jacobian jobs of orders 3-4 run in the batch's element-once groups, one job per plan and batch; a
+    second job on the same plan takes the owner path. LINEAR strain: K does not depend on the states and
+    R(2u) = 2 R(u). Replayed after clear and refill (graph batches read their tables from device memory)."""
+    torch = torch_gpu
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")  # noqa: E731
+    high = [c for c in meshes if c["mesh"].order >= 3]
+    plans = [fem.Plan(c["mesh"], c["pat"]) for c in high]
+    batch = fem.Batch(graph=graph)
+    for rnd in range(2):
+        batch.clear()
+        jobs = []
+        for i, (c, plan) in enumerate(zip(high, plans)):
+            m = c["mesh"]
+            coords = d(m.coords)
+            for k, scale in enumerate((1.0, 2.0)):  # k = 1: the same plan again
+                states = d(scale * c["u"])
+                with_res = (i + k + rnd) % 2 == 0
+                res = torch.full((2 * m.num_nodes,), float("nan"), dtype=torch.float64,
+                                 device="cuda:0") if with_res else None
+                vals = torch.full((4 * c["pat"].num_blocks,), float("nan"), dtype=torch.float64, device="cuda:0")
+                batch.add(plan, states, coords, E, NU, T, res=res, values=vals)
+                jobs.append((c, scale, res, vals, states, coords))
+        batch.assemble()
+        torch.cuda.synchronize()
+        for c, scale, res, vals, _, _ in jobs:
+            assert textbook.normwise_rel(vals.cpu().numpy(), c["vals"]) < 1e-12
+            if res is not None:
+                assert textbook.normwise_rel(res.cpu().numpy(), scale * c["res"]) < 1e-12
+    # the element-once results are bitwise those of the plan assembled alone
+    for (c, scale, res, vals, states, coords), plan in zip(jobs[::2], plans):
+        alone = torch.zeros_like(vals)
+        plan.assemble(states, coords, E, NU, T, values=alone)
+        torch.cuda.synchronize()
+        assert torch.equal(alone, vals)
+    batch.close()
+    for plan in plans:
+        plan.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_element_once_many_units(fem, orc, torch_gpu, graph):
+    """Element-once jobs with more units than the persistent grid (each workgroup loops) and element counts
+    that leave the last unit's second wavefront without elements (8281 = 4 * 2070 + 1 elements at order 4,
+    two per wavefront; 16900 = 8 * 2112 + 4 at order 3, four per wavefront). Against the oracle and, bitwise,
+    the plan assembled alone."""
+    torch = torch_gpu
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")  # noqa: E731
+    cases = []
+    for p, n in ((4, 91), (3, 130)):
+        m = fem.square_mesh(n, p, jitter=0.2, seed=p)
+        u = fem.fill_uniform(2 * m.num_nodes, -1e-3, 1e-3, 40 + p)
+        pat = fem.bsr_pattern(m)
+        res_ref, vals_ref = orc.jacobian(m, u, E, NU, T, fem.element_bcsr_map(m, pat), pat.num_blocks)
+        cases.append((m, pat, fem.Plan(m, pat), d(u), d(m.coords), res_ref, vals_ref))
+    batch = fem.Batch(graph=graph)
+    outs = []
+    for m, pat, plan, states, coords, _, _ in cases:
+        res = torch.full((2 * m.num_nodes,), float("nan"), dtype=torch.float64, device="cuda:0")
+        vals = torch.full((4 * pat.num_blocks,), float("nan"), dtype=torch.float64, device="cuda:0")
+        batch.add(plan, states, coords, E, NU, T, res=res, values=vals)
+        outs.append((res, vals))
+    for _ in range(2):
+        batch.assemble()
+    torch.cuda.synchronize()
+    for (m, pat, plan, states, coords, res_ref, vals_ref), (res, vals) in zip(cases, outs):
+        assert textbook.normwise_rel(res.cpu().numpy(), res_ref) < 1e-12
+        assert textbook.normwise_rel(vals.cpu().numpy(), vals_ref) < 1e-12
+        r1 = torch.full_like(res, float("nan"))
+        v1 = torch.full_like(vals, float("nan"))
+        plan.assemble(states, coords, E, NU, T, res=r1, values=v1)
+        torch.cuda.synchronize()
+        assert torch.equal(r1, res) and torch.equal(v1, vals)
+        plan.close()
+    batch.close()
