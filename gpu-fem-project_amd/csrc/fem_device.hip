@@ -771,8 +771,11 @@ __device__ __forceinline__ int find_job(const int *start, int n, int64_t x) {
 // A batch's element-once jobs of one order: counts = (jobs, units, staged blocks, staged residual entries)
 // in device memory (so a recorded graph replays any batch), unitStart[j] the first unit of job j, a unit
 // being WAVES * EPW elements. Persistent: workgroups stride over the units.
+// Two wavefronts per SIMD are requested explicitly: the persistent loop's state would otherwise push the
+// FP64 variants past 256 registers (one wavefront per SIMD); FP64 batch 56.6 ms against 59.3 ms with a few
+// registers spilled.
 template <int P, typename T, int WAVES, bool RES>
-__global__ __launch_bounds__(WAVES * 64) void element_tiled_batch(const SplitJob *__restrict__ jobs,
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2))) void element_tiled_batch(const SplitJob *__restrict__ jobs,
                                                                    const int *__restrict__ unitStart,
                                                                    const int *__restrict__ counts,
                                                                    const Basis1D basisIn) {
