@@ -720,6 +720,48 @@ int fem_bsr_pattern_cols(int numNodes, int numElements, int nodesPerElement, con
   return FEM_OK;
 }
 
+int fem_split_tables(int order, int numNodes, int numElements, const int *connPtr, const int *conn, const int *rowp,
+                     const int *cols, int *code, int *rcode, int *gdst, int *gptr, int *rdst, int *rptr,
+                     int64_t *numSlots, int64_t *numGather, int64_t *numRslots, int64_t *numRgather) {
+  if (!valid_order(order)) return set_error(FEM_ERR_INVALID_ORDER, "element order %d outside 1..4", order);
+  if (!rowp || !cols || !code || !rcode || !gdst || !gptr || !rdst || !rptr) {
+    return set_error(FEM_ERR_NULL_POINTER, "an input or output array is NULL");
+  }
+  const int N = nodes_per_element(order), RP = rank_pad(order);
+  int rc = check_connectivity(numNodes, numElements, N, connPtr, conn);
+  if (rc) return rc;
+  for (int n = 0; n < numNodes; n++) {
+    if (rowp[n + 1] - rowp[n] > 256) return set_error(FEM_ERR_UNSUPPORTED, "BSR row %d exceeds 256 blocks", n);
+  }
+  // element-major connectivity and the plan's column ranks, as fem_plan_create forms them
+  std::vector<int> c(size_t(numElements) * N);
+  std::vector<uint8_t> rank(size_t(numElements) * N * RP, 0);
+  for (int e = 0; e < numElements; e++) {
+    const int *en = conn + connPtr[e];
+    std::copy(en, en + N, c.begin() + size_t(e) * N);
+    for (int i = 0; i < N; i++) {
+      for (int j = 0; j < N; j++) {
+        const int pos = find_sorted(cols, rowp[en[i]], rowp[en[i] + 1], en[j]);
+        if (pos < 0) return set_error(FEM_ERR_MAP_MISS, "element %d couples nodes with no block in the pattern", e);
+        rank[(size_t(e) * N + j) * RP + i] = uint8_t(pos - rowp[en[i]]);
+      }
+    }
+  }
+  SplitHost sp;
+  if ((rc = build_split_host(order, numNodes, numElements, c.data(), rowp, rank, rowp[numNodes], sp))) return rc;
+  std::copy(sp.code.begin(), sp.code.end(), code);
+  std::copy(sp.rcode.begin(), sp.rcode.end(), rcode);
+  std::copy(sp.dst.begin(), sp.dst.end(), gdst);
+  std::copy(sp.ptr.begin(), sp.ptr.end(), gptr);
+  std::copy(sp.rdst.begin(), sp.rdst.end(), rdst);
+  std::copy(sp.rptr.begin(), sp.rptr.end(), rptr);
+  if (numSlots) *numSlots = sp.slots;
+  if (numGather) *numGather = int64_t(sp.dst.size());
+  if (numRslots) *numRslots = sp.rslots;
+  if (numRgather) *numRgather = int64_t(sp.rdst.size());
+  return FEM_OK;
+}
+
 int fem_element_bcsr_map(const int *connPtr, const int *conn, int numElements, int nodesPerElement, int numRows,
                          const int *rowp, const int *cols, int bsize, int *map) {
   if (!rowp || !cols || !map) return set_error(FEM_ERR_NULL_POINTER, "rowp/cols/map is NULL");

@@ -62,6 +62,7 @@ SIGNATURES = {
     "fem_bsr_pattern_rowp": ([_I, _I, _I, _P, _P, _P], _I),
     "fem_bsr_pattern_cols": ([_I, _I, _I, _P, _P, _P, _P], _I),
     "fem_element_bcsr_map": ([_P, _P, _I, _I, _I, _P, _P, _I, _P], _I),
+    "fem_split_tables": ([_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _PI64, _PI64, _PI64, _PI64], _I),
     "fem_bsr_pattern_device": ([_I, _I, _I, _P, _P, _P, _P, _I64, _PI64, _P], _I),
     "fem_write_array": ([_P, _I64, ctypes.c_char_p], _I),
     "fem_write_bsr_matrix": ([_I, _I, _I, _P, _P, _P, _I, _I, ctypes.c_char_p], _I),
@@ -271,6 +272,25 @@ def element_bcsr_map(mesh: Mesh, pattern: Pattern, bsize: int = 2) -> np.ndarray
     _check(load().fem_element_bcsr_map(_ptr(cp), _ptr(mesh.conn), mesh.num_elements, mesh.npe, mesh.num_nodes,
                                        _ptr(pattern.rowp), _ptr(pattern.cols), bsize, _ptr(out)))
     return out
+
+
+def split_tables(mesh: Mesh, pattern: Pattern) -> dict:
+    """The element-once tables a plan builds for Jacobian assemblies (fem_split_tables), on the host:
+    block codes [E][N][N], residual codes [E][N], and the gather lists of the non-exclusive outputs."""
+    E, N, nn, nb = mesh.num_elements, mesh.npe, mesh.num_nodes, pattern.num_blocks
+    code = np.zeros(E * N * N, dtype=np.int32)
+    rcode = np.zeros(E * N, dtype=np.int32)
+    gdst = np.zeros(max(nb, 1), dtype=np.int32)
+    gptr = np.zeros(nb + 1, dtype=np.int32)
+    rdst = np.zeros(max(nn, 1), dtype=np.int32)
+    rptr = np.zeros(nn + 1, dtype=np.int32)
+    counts = [ctypes.c_int64() for _ in range(4)]
+    _check(load().fem_split_tables(mesh.order, nn, E, _ptr(mesh.conn_ptr), _ptr(mesh.conn), _ptr(pattern.rowp),
+                                   _ptr(pattern.cols), _ptr(code), _ptr(rcode), _ptr(gdst), _ptr(gptr), _ptr(rdst),
+                                   _ptr(rptr), *[ctypes.byref(c) for c in counts]))
+    ng, nrg = counts[1].value, counts[3].value
+    return dict(code=code, rcode=rcode, gdst=gdst[:ng], gptr=gptr[:ng + 1], rdst=rdst[:nrg], rptr=rptr[:nrg + 1],
+                slots=counts[0].value, rslots=counts[2].value)
 
 
 def write_array(values: np.ndarray, path: str) -> None:

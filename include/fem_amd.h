@@ -101,6 +101,17 @@ int fem_bsr_pattern_cols(int numNodes, int numElements, int nodesPerElement, con
  * (FEM_ERR_MAP_MISS), not a printed warning with the entry left unset (TACSHelpers.h:176-178). */
 int fem_element_bcsr_map(const int *connPtr, const int *conn, int numElements, int nodesPerElement, int numRows,
                          const int *rowp, const int *cols, int bsize, int *map);
+/* Element-once tables of a mesh on the host: what fem_plan_create builds for Jacobian assemblies of orders
+ * 3-4 (any order here). code[(e*N + a)*N + b] is the BSR index of block (conn[e,a], conn[e,b]) when element e
+ * alone forms it, else numBlocks + a staging slot; rcode[e*N + a] likewise for node conn[e,a]'s residual,
+ * against numNodes. gdst / gptr list the other blocks (ascending) and their slot ranges (gptr has
+ * *numGather + 1 entries), rdst / rptr the other nodes. Each output is the sum of its slots in slot order
+ * (zero if none). Array sizes: code E*N*N, rcode E*N, gdst numBlocks, gptr numBlocks + 1, rdst numNodes,
+ * rptr numNodes + 1 (upper bounds). No counterpart in the reference (its kernels add with atomics); exposed
+ * so that the tables behind the plans can be checked on the host. */
+int fem_split_tables(int order, int numNodes, int numElements, const int *connPtr, const int *conn, const int *rowp,
+                     const int *cols, int *code, int *rcode, int *gdst, int *gptr, int *rdst, int *rptr,
+                     int64_t *numSlots, int64_t *numGather, int64_t *numRslots, int64_t *numRgather);
 /* ---- Output files in the reference's formats (host arrays) -----------------------------------------
  * fem_write_array: one value per line, "% .17g" (writeArrayToFile, verification/TACSHelpers.h:234-246).
  * fem_write_bsr_matrix: header "rows cols nnz" in scalar entries, then "row col value" per entry, 0-based,
