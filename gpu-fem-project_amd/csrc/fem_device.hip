@@ -523,6 +523,36 @@ struct TileMap {
   static_assert(Ord<P>::EPW * T <= 64, "the tiles of a wavefront's elements exceed its lanes");
 };
 
+// The emission's store instructions: group R of 4 rows takes iters(R) instructions (16 blocks each over
+// the wavefront's elements), issued in chunks of CHK whose destination codes are loaded one chunk ahead
+// (order 3: 16 instructions per group in two chunks, so the codes fit the registers beside the Gram tiles).
+template <int P>
+struct EmitPlan {
+  static constexpr int N = Ord<P>::N, EPW = Ord<P>::EPW, G = TileMap<P>::G;
+  static constexpr int CHK = P == 3 ? 8 : 16;
+  static constexpr int iters(int R) { return (EPW * (N - 4 * R < 4 ? N - 4 * R : 4) * N + 15) / 16; }
+  static constexpr int chunks(int R) { return (iters(R) + CHK - 1) / CHK; }
+  static constexpr int steps() {
+    int t = 0;
+    for (int R = 0; R < G; R++) t += chunks(R);
+    return t;
+  }
+  static constexpr int group(int k) {
+    for (int R = 0; R < G; R++) {
+      if (k < chunks(R)) return R;
+      k -= chunks(R);
+    }
+    return G - 1;
+  }
+  static constexpr int chunk(int k) {
+    for (int R = 0; R < G; R++) {
+      if (k < chunks(R)) return k;
+      k -= chunks(R);
+    }
+    return 0;
+  }
+};
+
 template <int P, typename T>
 struct __align__(16) WaveLDSTile {
   using O = Ord<P>;
@@ -674,11 +704,13 @@ __device__ __forceinline__ void tiled_wave(WaveLDSTile<P, T> &W, const Basis1D &
   // Emission, one group of 4 rows at a time: the tiles holding blocks of rows 4R..4R+3 write them to LDS
   // (tile (R, J) directly; tile (J, R), J < R, transposed: K(b, a) = K(a, b)^T), then each store instruction
   // covers the four components of 16 consecutive blocks of the group's rows (lanes 4k..4k+3: block k). The
-  // destination codes of group R+1 are loaded while group R is stored. Every code load and store is
-  // unconditional, so that the wait for a group's codes lets the previous group's stores stay in flight:
+  // destination codes of the next chunk of stores (EmitPlan) are loaded while the current chunk is stored.
+  // Every code load and store is unconditional, so that the wait for a chunk's codes lets the previous
+  // chunk's stores stay in flight:
   // lanes past the group's last block (or past the wavefront's last element) repeat the store of that block,
   // the same value to the same address in the same instruction.
-  constexpr int ITER = (EPW * 4 * N + 15) / 16;  // store instructions of a full group of 4 rows
+  using EP = EmitPlan<P>;
+  constexpr int CHK = EP::CHK;
   const int c = lane & 3;
   const int lastEl = int(min<int64_t>(EPW, job.count - first)) - 1;
   // block (e, rem) of store instruction `it` of group R (rem: offset within the element's rows of the group)
@@ -698,48 +730,51 @@ __device__ __forceinline__ void tiled_wave(WaveLDSTile<P, T> &W, const Basis1D &
       rem = groupBlocks - 1;
     }
   };
-  int cur[ITER], nxt[ITER];
-  auto load_codes = [&](auto Rc, int (&at)[ITER]) __attribute__((always_inline)) {
-    constexpr int R = decltype(Rc)::value;
-    constexpr int IT = (EPW * (N - 4 * R < 4 ? N - 4 * R : 4) * N + 15) / 16;
+  int cur[CHK], nxt[CHK];
+  // codes of step k: chunk EP::chunk(k) of group EP::group(k)
+  auto load_codes = [&](auto Kc, int (&at)[CHK]) __attribute__((always_inline)) {
+    constexpr int k = decltype(Kc)::value, R = EP::group(k), ch = EP::chunk(k);
 #pragma unroll
-    for (int it = 0; it < IT; it++) {
+    for (int it = 0; it < CHK; it++) {
+      if (ch * CHK + it >= EP::iters(R)) break;
       int e, rem;
-      locate(R, it, e, rem);
+      locate(R, ch * CHK + it, e, rem);
       at[it] = job.code[((first + e) * N + 4 * R) * N + rem];
     }
   };
   load_codes(std::integral_constant<int, 0>{}, cur);
-  static_for<TM::G>([&](auto Rc) __attribute__((always_inline)) {
-    constexpr int R = decltype(Rc)::value;
-    constexpr int IT = (EPW * (N - 4 * R < 4 ? N - 4 * R : 4) * N + 15) / 16;
-    if constexpr (R + 1 < TM::G) load_codes(std::integral_constant<int, R + 1>{}, nxt);
-    wave_sync();  // the previous group's (or the last point's) LDS reads are done
-    if (tvalid && I == R) {
+  static_for<EP::steps()>([&](auto Kc) __attribute__((always_inline)) {
+    constexpr int k = decltype(Kc)::value, R = EP::group(k), ch = EP::chunk(k);
+    if constexpr (k + 1 < EP::steps()) load_codes(std::integral_constant<int, k + 1>{}, nxt);
+    if constexpr (ch == 0) {
+      wave_sync();  // the previous group's (or the last point's) LDS reads are done
+      if (tvalid && I == R) {
 #pragma unroll
-      for (int i = 0; i < 4; i++)
+        for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++)
-          if (4 * I + i < N && 4 * J + j < N) W.put(tec, i, 4 * J + j, S[i][j][0], S[i][j][1], S[i][j][2], S[i][j][3]);
-    } else if (tvalid && J == R) {
+          for (int j = 0; j < 4; j++)
+            if (4 * I + i < N && 4 * J + j < N) W.put(tec, i, 4 * J + j, S[i][j][0], S[i][j][1], S[i][j][2], S[i][j][3]);
+      } else if (tvalid && J == R) {
 #pragma unroll
-      for (int i = 0; i < 4; i++)
+        for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++)
-          if (4 * J + j < N) W.put(tec, j, 4 * I + i, S[i][j][0], S[i][j][2], S[i][j][1], S[i][j][3]);
+          for (int j = 0; j < 4; j++)
+            if (4 * J + j < N) W.put(tec, j, 4 * I + i, S[i][j][0], S[i][j][2], S[i][j][1], S[i][j][3]);
+      }
+      wave_sync();
     }
-    wave_sync();
 #pragma unroll
-    for (int it = 0; it < IT; it++) {
+    for (int it = 0; it < CHK; it++) {
+      if (ch * CHK + it >= EP::iters(R)) break;
       int e, rem;
-      locate(R, it, e, rem);
+      locate(R, ch * CHK + it, e, rem);
       const int at = cur[it];
       const int i = rem / N, b = rem - i * N;
       double *blkp = at < job.numBlocks ? job.values + 4 * int64_t(at) : job.stage + 4 * (int64_t(at) - job.numBlocks);
       __builtin_nontemporal_store(double(W.get(e, i, b, c)), blkp + c);
     }
 #pragma unroll
-    for (int it = 0; it < ITER; it++) cur[it] = nxt[it];
+    for (int it = 0; it < CHK; it++) cur[it] = nxt[it];
   });
 }
 
