@@ -2700,6 +2700,8 @@ int create_plan(int order, int numNodes, int numElements, const int *connPtr, co
       }
       p->numGather = int(sp.dst.size());
       p->numRgather = int(sp.rdst.size());
+      p->numSlots = sp.slots;
+      p->numRslots = sp.rslots;
     } else {
       clear_error();
     }
@@ -2884,12 +2886,16 @@ struct fem_batch {
   struct SplitGroup {
     std::vector<SplitJob> jobs;
     std::vector<int> unitStart{0}, gStart{0}, rStart{0};  // prefix sums: units, staged blocks, staged residuals
+    std::vector<int64_t> stageOff, rstageOff;             // each job's staging, in doubles into the arena
   };
   static constexpr int kGroups = 12;
   static constexpr int kSplitGroups = 4;
   Group groups[kGroups];
   SplitGroup split[kSplitGroups];
-  std::vector<const fem_plan *> splitPlans;  // plans with a job in `split` (their staging buffers are in use)
+  // Staging of the element-once jobs: the batch's own (so that batches holding one plan can run
+  // concurrently), grown at assembly to the jobs' total and kept.
+  double *dArena = nullptr;
+  int64_t arenaDoubles = 0, splitDoubles = 0;
   int maxJobs = 0;
   int flags = 0;
   int numCUs = 0;
@@ -2932,6 +2938,7 @@ void free_batch(fem_batch *b) {
     (void)hipEventDestroy(b->staged);
   }
   if (b->captureStream) (void)hipStreamDestroy(b->captureStream);
+  if (b->dArena) (void)hipFree(b->dArena);
   if (b->dTables) (void)hipFree(b->dTables);
   if (b->hTables) (void)hipHostFree(b->hTables);
 }
@@ -3056,8 +3063,10 @@ int fem_batch_clear(fem_batch *batch) {
     g.unitStart.assign(1, 0);
     g.gStart.assign(1, 0);
     g.rStart.assign(1, 0);
+    g.stageOff.clear();
+    g.rstageOff.clear();
   }
-  batch->splitPlans.clear();
+  batch->splitDoubles = 0;
   batch->numJobs = 0;
   return FEM_OK;
 }
@@ -3073,10 +3082,9 @@ int fem_batch_add(fem_batch *batch, fem_plan *plan, const double *states, const 
   Material m;
   int rc = make_material(E, nu, t, m);
   if (rc) return rc;
-  // Jacobian jobs of plans with element-once tables take the element-once groups, one job per plan and
-  // batch (the plan's staging buffer); a further job on the same plan takes the owner path.
-  if (plan->dConn && matEntries &&
-      std::find(batch->splitPlans.begin(), batch->splitPlans.end(), plan) == batch->splitPlans.end()) {
+  // Jacobian jobs of plans with element-once tables take the element-once groups; their staging is placed
+  // in the batch's arena at assembly.
+  if (plan->dConn && matEntries) {
     fem_batch::SplitGroup &g = batch->split[(plan->order - 3) * 2 + (residual ? 1 : 0)];
     if (int(g.jobs.size()) >= batch->maxJobs) {
       return set_error(FEM_ERR_INVALID_ARGUMENT, "batch group is full (%d jobs)", batch->maxJobs);
@@ -3093,7 +3101,10 @@ int fem_batch_add(fem_batch *batch, fem_plan *plan, const double *states, const 
     g.unitStart.push_back(int(units));
     g.gStart.push_back(int(staged));
     g.rStart.push_back(int(rstaged));
-    batch->splitPlans.push_back(plan);
+    g.stageOff.push_back(batch->splitDoubles);
+    batch->splitDoubles += 4 * plan->numSlots;
+    g.rstageOff.push_back(batch->splitDoubles);
+    if (residual) batch->splitDoubles += 2 * plan->numRslots;
     batch->numJobs++;
     return FEM_OK;
   }
@@ -3123,6 +3134,29 @@ int fem_batch_assemble(fem_batch *batch, int precision, void *stream, double *el
   if (batch->stagedPending) {
     FEM_HIP_CHECK(hipEventSynchronize(batch->staged));
     batch->stagedPending = false;
+  }
+  // The element-once jobs' staging arena: grown (after the device has drained, so no earlier assembly
+  // still uses the old one) and never shrunk.
+  if (batch->splitDoubles > batch->arenaDoubles) {
+    if (batch->dArena) {
+      FEM_HIP_CHECK(hipDeviceSynchronize());
+      FEM_HIP_CHECK(hipFree(batch->dArena));
+    }
+    batch->dArena = nullptr;
+    batch->arenaDoubles = 0;
+    if (hipMalloc(reinterpret_cast<void **>(&batch->dArena), size_t(batch->splitDoubles) * sizeof(double)) !=
+        hipSuccess) {
+      batch->dArena = nullptr;
+      return set_error(FEM_ERR_OUT_OF_MEMORY, "allocating %lld doubles of batch staging failed",
+                       (long long)batch->splitDoubles);
+    }
+    batch->arenaDoubles = batch->splitDoubles;
+  }
+  for (auto &g : batch->split) {
+    for (size_t j = 0; j < g.jobs.size(); j++) {
+      g.jobs[j].stage = batch->dArena + g.stageOff[j];
+      g.jobs[j].rstage = batch->dArena + g.rstageOff[j];
+    }
   }
   // Counts of all groups and job tables of non-empty groups, staged in the mirror and copied up to the
   // end of the last non-empty group's jobs.

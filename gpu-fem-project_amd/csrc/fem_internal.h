@@ -155,6 +155,7 @@ struct fem_plan {
   double *dStage = nullptr;  // [4 * slots]
   double *dRstage = nullptr; // [2 * rslots]
   int numGather = 0, numRgather = 0;
+  int64_t numSlots = 0, numRslots = 0;  // staged contributions: blocks, residual entries
   double partScale = 1.0;    // part budgets relative to the kernel's (FEM_PLAN_AUTOTUNE picks it)
   double tunedSeconds = 0;   // measured assembly time of the chosen build (autotuned plans)
   void *dJob = nullptr;      // device job record of the last assembly

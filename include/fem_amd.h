@@ -214,8 +214,9 @@ int fem_plan_destroy(fem_plan *plan);
  * Many assemblies on many resident plans in one call: jobs are grouped by element order and outputs
  * and each group runs as ONE persistent launch over all of its meshes' parts (no per-mesh launches).
  * Jacobian jobs of orders 3-4 run element-once (see fem_plan_create): one persistent launch per order
- * and residual/no-residual group, then the gathers; a plan takes part in at most one such job per batch
- * (its staging buffer), a further job on the same plan runs on the owner parts.
+ * and residual/no-residual group, then the gathers, with staging in the batch's own device arena (grown
+ * at assembly when the batch's jobs need more, after a device synchronisation), so batches holding the
+ * same plans may run concurrently on different streams.
  * Pointers are the same as fem_plan_assemble's; outputs are overwritten. Plans must be built with
  * flags 0. fem_batch_assemble uploads the job tables (asynchronously on `stream`) and launches; the
  * batch may be cleared and refilled once the call returns. */

@@ -97,8 +97,8 @@ def test_batch_rejects_deterministic_plans(fem, torch_gpu, meshes):
 
 @pytest.mark.parametrize("graph", [False, True])
 def test_batch_element_once_groups(fem, torch_gpu, meshes, graph):
-    """Jacobian jobs of orders 3-4 run in the batch's element-once groups, one job per plan and batch; a
-    second job on the same plan takes the owner path. LINEAR strain: K does not depend on the states and
+    """Jacobian jobs of orders 3-4 run in the batch's element-once groups, each with its own staging in the
+    batch's arena, also two jobs on one plan. LINEAR strain: K does not depend on the states and
     R(2u) = 2 R(u). Replayed after clear and refill (graph batches read their tables from device memory)."""
     torch = torch_gpu
     d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")  # noqa: E731
@@ -171,3 +171,36 @@ def test_batch_element_once_many_units(fem, orc, torch_gpu, graph):
         assert torch.equal(r1, res) and torch.equal(v1, vals)
         plan.close()
     batch.close()
+
+
+def test_batches_sharing_plans_run_concurrently(fem, torch_gpu, meshes):
+    """Two batches holding the same order 3-4 plans, assembled on two streams without synchronisation in
+    between (as the serving loop's slots do): each batch stages into its own arena, so both results are the
+    oracle's (states u and 3u: K unchanged, R scaled)."""
+    torch = torch_gpu
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")  # noqa: E731
+    high = [c for c in meshes if c["mesh"].order >= 3]
+    plans = [fem.Plan(c["mesh"], c["pat"]) for c in high]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    batches, outs = [], []
+    for k, scale in enumerate((1.0, 3.0)):
+        b = fem.Batch(graph=bool(k))
+        for c, plan in zip(high, plans):
+            m = c["mesh"]
+            res = torch.full((2 * m.num_nodes,), float("nan"), dtype=torch.float64, device="cuda:0")
+            vals = torch.full((4 * c["pat"].num_blocks,), float("nan"), dtype=torch.float64, device="cuda:0")
+            b.add(plan, d(scale * c["u"]), d(m.coords), E, NU, T, res=res, values=vals)
+            outs.append((c, scale, res, vals))
+        batches.append(b)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        for b, st in zip(batches, streams):
+            b.assemble(stream=st)
+    torch.cuda.synchronize()
+    for c, scale, res, vals in outs:
+        assert textbook.normwise_rel(vals.cpu().numpy(), c["vals"]) < 1e-12
+        assert textbook.normwise_rel(res.cpu().numpy(), scale * c["res"]) < 1e-12
+    for b in batches:
+        b.close()
+    for plan in plans:
+        plan.close()
