@@ -2233,7 +2233,9 @@ void launch_element_order(int order, const ElemArgs &a, int precision, hipStream
   }
 }
 
-constexpr int kTiledWaves = 2;  // wavefronts per workgroup of the element-once kernels
+// Wavefronts per workgroup of the element-once kernels (order-4 annulus, FP64: 2 waves 3.13 ms, 4 waves
+// 2.95 ms, 8 waves 3.05 ms; the workgroup's basis tables are formed once for more elements).
+constexpr int kTiledWaves = 4;
 
 template <int P, typename T, bool RES>
 void launch_element_split(const SplitJob &job, const Basis1D &basis, hipStream_t stream) {

@@ -138,14 +138,15 @@ def test_batch_element_once_groups(fem, torch_gpu, meshes, graph):
 
 @pytest.mark.parametrize("graph", [False, True])
 def test_batch_element_once_many_units(fem, orc, torch_gpu, graph):
-    """Element-once jobs with more units than the persistent grid (each workgroup loops) and element counts
-    that leave the last unit's second wavefront without elements (8281 = 4 * 2070 + 1 elements at order 4,
-    two per wavefront; 16900 = 8 * 2112 + 4 at order 3, four per wavefront). Against the oracle and, bitwise,
-    the plan assembled alone."""
+    """Element-once jobs with more units (four wavefronts' elements) than the persistent grid of 8
+    workgroups per CU on 256 CUs (each workgroup loops), and element counts that leave wavefronts of the
+    last unit without elements (16900 = 8 * 2112 + 4 elements at order 4, two per wavefront; 33124 =
+    16 * 2070 + 4 at order 3, four per wavefront). Against the oracle and, bitwise, the plan assembled
+    alone."""
     torch = torch_gpu
     d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")  # noqa: E731
     cases = []
-    for p, n in ((4, 91), (3, 130)):
+    for p, n in ((4, 130), (3, 182)):
         m = fem.square_mesh(n, p, jitter=0.2, seed=p)
         u = fem.fill_uniform(2 * m.num_nodes, -1e-3, 1e-3, 40 + p)
         pat = fem.bsr_pattern(m)
