@@ -789,7 +789,11 @@ __global__ __launch_bounds__(WAVES * 64) void element_tiled_kernel(const SplitJo
   fill_gref<P>(gref, basisIn);
   __syncthreads();
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  tiled_wave<P, T, RES>(wl[wave], basis, gref, job, (int64_t(blockIdx.x) * WAVES + wave) * O::EPW, lane);
+  // Workgroup b runs on XCD b % 8; each XCD takes a contiguous range of units, so neighbouring elements
+  // (which write neighbouring blocks of shared BSR rows) meet in the same L2.
+  const int nb = int(gridDim.x), q = nb / 8, r = nb % 8, x = int(blockIdx.x) % 8;
+  const int unit = x * q + min(x, r) + int(blockIdx.x) / 8;
+  tiled_wave<P, T, RES>(wl[wave], basis, gref, job, (int64_t(unit) * WAVES + wave) * O::EPW, lane);
 }
 
 // Index j of the last start[j] <= x over start[0..n) (start ascending, start[0] = 0 <= x).
