@@ -172,6 +172,14 @@ __device__ __forceinline__ int rank_of(const WL &W, int el, int row, int col) {
 
 __device__ __forceinline__ void wave_sync() { __syncwarp(); }
 
+// Workgroup index remapped so that the workgroups of one XCD (hardware placement: workgroup b on XCD
+// b % 8) take a contiguous range of indices: neighbouring elements, whose outputs share cache lines, then
+// meet in one L2.
+__device__ __forceinline__ int xcd_block() {
+  const int nb = int(gridDim.x), q = nb / 8, r = nb % 8, x = int(blockIdx.x) % 8;
+  return x * q + min(x, r) + int(blockIdx.x) / 8;
+}
+
 __device__ __forceinline__ void copy_basis(Basis1D &dst, const Basis1D &src) {
   const double *s = reinterpret_cast<const double *>(&src);
   double *d = reinterpret_cast<double *>(&dst);
@@ -433,7 +441,7 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel(const ElemArgs args
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int el = lane / N, idx = lane % N;
   WaveLDS<P, T> &W = wl[wave];
-  const int64_t slot = (int64_t(blockIdx.x) * WAVES + wave) * O::EPW + el;
+  const int64_t slot = (int64_t(xcd_block()) * WAVES + wave) * O::EPW + el;
   const bool valid = lane < O::LANES && slot < args.count;
   int eid = 0, node = 0;
   if (valid) {
@@ -789,11 +797,7 @@ __global__ __launch_bounds__(WAVES * 64) void element_tiled_kernel(const SplitJo
   fill_gref<P>(gref, basisIn);
   __syncthreads();
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  // Workgroup b runs on XCD b % 8; each XCD takes a contiguous range of units, so neighbouring elements
-  // (which write neighbouring blocks of shared BSR rows) meet in the same L2.
-  const int nb = int(gridDim.x), q = nb / 8, r = nb % 8, x = int(blockIdx.x) % 8;
-  const int unit = x * q + min(x, r) + int(blockIdx.x) / 8;
-  tiled_wave<P, T, RES>(wl[wave], basis, gref, job, (int64_t(unit) * WAVES + wave) * O::EPW, lane);
+  tiled_wave<P, T, RES>(wl[wave], basis, gref, job, (int64_t(xcd_block()) * WAVES + wave) * O::EPW, lane);
 }
 
 // Index j of the last start[j] <= x over start[0..n) (start ascending, start[0] = 0 <= x).
@@ -1010,7 +1014,7 @@ __global__ __launch_bounds__(WAVES * 64) void element_kernel_nl(const ElemArgs a
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int el = lane / N, idx = lane % N;
   WaveLDSNL<P> &W = wl[wave];
-  const int64_t slot = (int64_t(blockIdx.x) * WAVES + wave) * O::EPW + el;
+  const int64_t slot = (int64_t(xcd_block()) * WAVES + wave) * O::EPW + el;
   const bool valid = lane < O::LANES && slot < args.count;
   int eid = 0, node = 0;
   if (valid) {
