@@ -3148,19 +3148,26 @@ int fem_batch_assemble(fem_batch *batch, int precision, void *stream, double *el
   // The element-once jobs' staging arena: grown (after the device has drained, so no earlier assembly
   // still uses the old one) and never shrunk.
   if (batch->splitDoubles > batch->arenaDoubles) {
+    // at least doubled, so that a stream of growing batches synchronises the device only a few times
+    const int64_t want = std::max(batch->splitDoubles, 2 * batch->arenaDoubles);
     if (batch->dArena) {
       FEM_HIP_CHECK(hipDeviceSynchronize());
       FEM_HIP_CHECK(hipFree(batch->dArena));
     }
     batch->dArena = nullptr;
     batch->arenaDoubles = 0;
-    if (hipMalloc(reinterpret_cast<void **>(&batch->dArena), size_t(batch->splitDoubles) * sizeof(double)) !=
-        hipSuccess) {
+    for (const int64_t n : {want, batch->splitDoubles}) {
+      if (hipMalloc(reinterpret_cast<void **>(&batch->dArena), size_t(n) * sizeof(double)) == hipSuccess) {
+        batch->arenaDoubles = n;
+        break;
+      }
+      (void)hipGetLastError();
       batch->dArena = nullptr;
+    }
+    if (!batch->dArena) {
       return set_error(FEM_ERR_OUT_OF_MEMORY, "allocating %lld doubles of batch staging failed",
                        (long long)batch->splitDoubles);
     }
-    batch->arenaDoubles = batch->splitDoubles;
   }
   for (auto &g : batch->split) {
     for (size_t j = 0; j < g.jobs.size(); j++) {
