@@ -304,11 +304,14 @@ __device__ __forceinline__ void point_terms(const PointGeo &G, T &nx, T &ny, V2 
 // LDS latency overlaps the arithmetic. The two publication buffers alternate; a buffer is rewritten two
 // iterations after it was read, and the LDS executes one wavefront's accesses in order.
 template <int P, typename T, bool JAC, typename Where, typename Emit>
-__device__ __forceinline__ void stage_stiffness(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int el,
+__device__ __forceinline__ void stage_stiffness(WaveLDS<P, T> &W, const double2 *gref, const Material &m, int elIn,
                                                 int b, bool valid, double &Rx, double &Ry, Where &&where,
                                                 Emit &&emit) {
   using O = Ord<P>;
   constexpr int N = O::N, Q = O::Q, H = O::H, D = O::D;
+  // Lanes past the last element of the wavefront (el == EPW) compute on element EPW-1's data: their LDS
+  // reads stay inside the arrays, and `valid` masks their writes and emits.
+  const int el = min(elIn, O::EPW - 1);
   using V2 = typename Vec2<T>::type;
   const T mu = T(m.mu), lam = T(m.lambda), c1 = lam + 2 * mu;
   Rx = 0;
@@ -1569,190 +1572,14 @@ __device__ __forceinline__ void zero_planes(double *acc, int nb) {
   }
 }
 
-// ---- register-resident owner kernel (one wave per SIMD) ------------------------------------------------
-// Each lane keeps the reference gradients of its stiffness rows at every quadrature point in registers,
-// so the Gram stage needs no exchange between lanes: per point it reads only the element's geometry
-// (J^-1, s_q, s_q*sigma). With one wave per SIMD a workgroup owns all 160 KB of LDS, and parts take
-// several batches per wavefront.
-template <int P, typename T>
-struct __align__(16) FatWaveLDS {
-  using O = Ord<P>;
-  double nodal[O::EPW][O::NODAL_STRIDE];  // x, y, u, v of each element node
-  double geo[O::EPW][O::GEO_STRIDE];      // per q: J^-1 (row-major), s_q, s_q*sigma_xx, s_q*sigma_yy, s_q*sigma_xy
-  int own[O::EPW][O::N];                  // local row of node a in this part, or -1
-  uint32_t rk[O::EPW][O::N][O::RPAD / 4];  // rk[el][b] byte a = slot of column b within row a
-};
-
-// 1-D factors of the reference gradients of a lane's stiffness rows a = b + d (mod N) at the Gauss points:
-// g_a(q) = (dphi_ax(x_qx) phi_ay(x_qy), phi_ax(x_qx) dphi_ay(x_qy)).
-template <int P, typename T>
-struct RowBasis {
-  T fx[Ord<P>::D][Ord<P>::NP], dfx[Ord<P>::D][Ord<P>::NP], fy[Ord<P>::D][Ord<P>::NP], dfy[Ord<P>::D][Ord<P>::NP];
-};
-
-template <int P, typename T>
-__device__ __forceinline__ void fill_row_basis(const Basis1D &B, int b, RowBasis<P, T> &rb) {
-  using O = Ord<P>;
-#pragma unroll
-  for (int d = 0; d < O::D; d++) {
-    const int a = b + d < O::N ? b + d : b + d - O::N;
-    const int ay = a / O::NP, ax = a - ay * O::NP;
-#pragma unroll
-    for (int i = 0; i < O::NP; i++) {
-      rb.fx[d][i] = T(B.phi[ax][i]);
-      rb.dfx[d][i] = T(B.dphi[ax][i]);
-      rb.fy[d][i] = T(B.phi[ay][i]);
-      rb.dfy[d][i] = T(B.dphi[ay][i]);
-    }
-  }
-}
-
-// Residual entries of node b and (JAC) stiffness blocks for lane (el, b), with the same symmetric row
-// pairing and emit contract as stage_stiffness. The physical gradients of the lane's rows are formed from
-// the register table and the point's J^-1; the geometry of point q+1 is read while point q accumulates.
-template <int P, typename T, bool JAC, typename WL, typename Emit>
-__device__ __forceinline__ void stage_stiffness_fat(const WL &W, const RowBasis<P, T> &rb, const Material &m, int el,
-                                                    int b, bool valid, double &Rx, double &Ry, Emit &&emit) {
-  using O = Ord<P>;
-  constexpr int N = O::N, Q = O::Q, H = O::H, D = O::D;
-  const T mu = T(m.mu), lam = T(m.lambda), c1 = lam + 2 * mu;
-  Rx = 0;
-  Ry = 0;
-  T S[D][4];
-#pragma unroll
-  for (int i = 0; i < D; i++) S[i][0] = S[i][1] = S[i][2] = S[i][3] = 0;
-  int arow[D];
-#pragma unroll
-  for (int d = 0; d < D; d++) arow[d] = b + d < N ? b + d : b + d - N;
-  const bool lastPair = (N % 2 == 1) || b < H;  // even N: distance-N/2 pair taken by the lower lane
-  const double *geo = &W.geo[el < O::EPW ? el : O::EPW - 1][0];  // lanes past the last element read a real row
-
-  double2 ia[2], ib[2], sc[2], sd[2];  // geometry of two consecutive points
-  auto load = [&](int q, int slot) __attribute__((always_inline)) {
-    const double *g = geo + 8 * q;
-    ia[slot] = *reinterpret_cast<const double2 *>(g + 0);
-    ib[slot] = *reinterpret_cast<const double2 *>(g + 2);
-    sc[slot] = *reinterpret_cast<const double2 *>(g + 4);
-    sd[slot] = *reinterpret_cast<const double2 *>(g + 6);
-  };
-  load(0, 0);
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    if (q + 1 < Q) load(q + 1, (q + 1) % 2);
-    const int sl = q % 2;
-    const int qy = q / O::NP, qx = q - qy * O::NP;
-    const T i00 = T(ia[sl].x), i01 = T(ia[sl].y), i10 = T(ib[sl].x), i11 = T(ib[sl].y);
-    T nx[D], ny[D];
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      const T gx = rb.dfx[d][qx] * rb.fy[d][qy], gy = rb.fx[d][qx] * rb.dfy[d][qy];
-      nx[d] = gx * i00 + gy * i10;
-      ny[d] = gx * i01 + gy * i11;
-    }
-    Rx += sc[sl].y * double(nx[0]) + sd[sl].y * double(ny[0]);
-    Ry += sd[sl].y * double(nx[0]) + sd[sl].x * double(ny[0]);
-    if (JAC) {
-      const T s = T(sc[sl].x);
-      const T wx = s * nx[0], wy = s * ny[0];
-#pragma unroll
-      for (int d = 0; d < D; d++) {
-        S[d][0] = fma(nx[d], wx, S[d][0]);
-        S[d][1] = fma(ny[d], wy, S[d][1]);
-        S[d][2] = fma(nx[d], wy, S[d][2]);
-        S[d][3] = fma(ny[d], wx, S[d][3]);
-      }
-    }
-  }
-  if (JAC && valid) {
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      if (d == H && !lastPair) continue;
-      double k[4];
-      k[0] = double(c1 * S[d][0] + mu * S[d][1]);
-      k[1] = double(lam * S[d][2] + mu * S[d][3]);
-      k[2] = double(lam * S[d][3] + mu * S[d][2]);
-      k[3] = double(c1 * S[d][1] + mu * S[d][0]);
-      emit(arow[d], b, d, k);
-      if (d != 0) {
-        const double kt[4] = {k[0], k[2], k[1], k[3]};
-        emit(b, arow[d], D + d, kt);
-      }
-    }
-  }
-}
-
-template <int P, typename T, int MAXB, bool RES, bool JAC, bool DET>
-__device__ __forceinline__ void owner_batch_fat(const Material &mat, FatWaveLDS<P, T> &W, const Basis1D &basis,
-                                                const RowBasis<P, T> &rb, double *acc, double *racc, const int *lrows,
-                                                const int *lbase, int nr, const Gather<P> &g, bool valid, int el,
-                                                int idx) {
-  int own = -1;
-  if (valid) {
-    own = (g.lrow < nr && lrows[g.lrow] == g.node) ? g.lrow : -1;
-    store_nodal(W, el, idx, g.xy, g.uv);
-    W.own[el][idx] = own;
-#pragma unroll
-    for (int i = 0; i < Ord<P>::RPAD / 4; i++) W.rk[el][idx][i] = g.rk[i];
-  }
-  wave_sync();
-  // LDS accumulator offsets of this lane's emits (slot d: block (a, b); slot D + d: block (b, a)), read now
-  // so that their latency overlaps the geometry and Gram stages.
-  using O = Ord<P>;
-  // Two 16-bit offsets per register: slot d in the low half, slot D + d in the high half, 0xffff for none.
-  uint32_t off[O::D];
-  {
-    const int elc = el < O::EPW ? el : O::EPW - 1;
-    const uint32_t hiB = own >= 0 ? uint32_t(lbase[own]) << 16 : 0xffff0000u;
-#pragma unroll
-    for (int d = 0; d < O::D; d++) {
-      const int a = idx + d < O::N ? idx + d : idx + d - O::N;
-      const int oa = W.own[elc][a];
-      const uint32_t lo = oa >= 0 ? uint32_t(lbase[oa] + rank_of(W, elc, a, idx)) : 0xffffu;
-      const uint32_t hi = own >= 0 ? hiB + (uint32_t(rank_of(W, elc, idx, a)) << 16) : 0xffff0000u;
-      off[d] = lo | hi;
-    }
-  }
-  if (valid) stage_geometry<P, T>(W, basis, mat, el, idx);
-  wave_sync();
-  double Rx, Ry;
-  stage_stiffness_fat<P, T, JAC>(W, rb, mat, el, idx, valid, Rx, Ry, [&](int, int, int slot, const double(&k)[4]) __attribute__((always_inline)) {
-    const uint32_t w = off[slot < O::D ? slot : slot - O::D];
-    const uint32_t h = slot < O::D ? (w & 0xffffu) : (w >> 16);
-    if (h == 0xffffu) return;
-    const int o = int(h);
-    double *dst = acc + o;
-    if (DET) {
-#pragma unroll
-      for (int c = 0; c < 4; c++) dst[c * acc_plane<MAXB>()] += k[c];
-    } else {
-#pragma unroll
-      for (int c = 0; c < 4; c++) atomicAdd(dst + c * acc_plane<MAXB>(), k[c]);
-    }
-  });
-  if (RES && valid && own >= 0) {
-    if (DET) {
-      racc[2 * own] += double(Rx);
-      racc[2 * own + 1] += double(Ry);
-    } else {
-      atomicAdd(&racc[2 * own], double(Rx));
-      atomicAdd(&racc[2 * own + 1], double(Ry));
-    }
-  }
-  wave_sync();
-}
-
 // Per-part kernel of a single job (deterministic plans and parts that take several batches per
 // wavefront): one workgroup per part.
 template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC, bool DET>
 __global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args) {
   using O = Ord<P>;
-  // Register-resident configurations size their parts for one workgroup per CU: they use the
-  // register-resident Gram stage and its smaller per-wave staging here too.
-  constexpr bool FAT = owner_config(P).fat != 0;
-  using WLT = typename std::conditional<FAT, FatWaveLDS<P, T>, WaveLDS<P, T>>::type;
-  __shared__ WLT wl[WAVES];
+  __shared__ WaveLDS<P, T> wl[WAVES];
   __shared__ Basis1D basis;
-  __shared__ double2 gref[FAT ? 1 : O::Q * O::N];
+  __shared__ double2 gref[O::Q * O::N];
   __shared__ __align__(16) double acc[JAC ? 4 * acc_plane<MAXB>() : 2];
   __shared__ __align__(16) double racc[RES ? 2 * MAXR : 2];
   __shared__ int lrows[MAXR];
@@ -1786,18 +1613,12 @@ __global__ __launch_bounds__(WAVES * 64) void owner_kernel(const OwnerArgs args)
     for (int i = threadIdx.x; i < d.nr; i += blockDim.x) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
   }
   copy_basis(basis, args.basis);
-  if constexpr (!FAT) fill_gref<P>(gref, args.basis);
+  fill_gref<P>(gref, args.basis);
   __syncthreads();
-  RowBasis<P, T> rb;
-  if constexpr (FAT) fill_row_basis<P, T>(basis, idx, rb);
 
-  WLT &W = wl[wave];
+  WaveLDS<P, T> &W = wl[wave];
   auto batch = [&](const Gather<P> &g, bool ok) __attribute__((always_inline)) {
-    if constexpr (FAT) {
-      owner_batch_fat<P, T, MAXB, RES, JAC, DET>(mat, W, basis, rb, acc, racc, lrows, lbase, d.nr, g, ok, el, idx);
-    } else {
-      owner_batch<P, T, MAXB, RES, JAC, DET>(mat, W, basis, gref, acc, racc, lrows, lbase, d.nr, g, ok, el, idx);
-    }
+    owner_batch<P, T, MAXB, RES, JAC, DET>(mat, W, basis, gref, acc, racc, lrows, lbase, d.nr, g, ok, el, idx);
   };
   if (DET) {
     const int *off = J.colourOff + int64_t(part) * (J.numColours + 1);
@@ -1934,10 +1755,7 @@ __device__ __forceinline__ void stream_parts(const MeshJob &J, int p0, int p1, W
 // many jobs (a batch of resident meshes of one order). A persistent workgroup walks a contiguous range
 // of global parts, job by job; job fields and material sit in scalar registers for the job's parts.
 template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC>
-#ifndef FEM_STREAM_MIN_WAVES
-#define FEM_STREAM_MIN_WAVES 2  // waves per SIMD the streaming kernel is held to
-#endif
-__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(FEM_STREAM_MIN_WAVES, 2)))
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void owner_stream_kernel(const OwnerArgs args) {
   using O = Ord<P>;
   static_assert(MAXR <= WAVES * 64, "one owned row per thread");
@@ -1983,149 +1801,6 @@ void owner_stream_kernel(const OwnerArgs args) {
     if (je > gp) {
       stream_parts<P, T, WAVES, MAXB, MAXR, RES, JAC>(J, gp - js, je - js, W, basis, gref, acc, racc, lrows, lbase,
                                                       lruns);
-      gp = je;
-    }
-  }
-}
-
-// Parts [p0, p1) of one job, each taking ceil(visits / (WAVES*EPW)) batches per wavefront. The batches
-// form one software pipeline across part boundaries: while batch s computes, batch s+1's coordinate,
-// state and rank gathers and batch s+2's node ids are in flight; part p+1's row tables load during part
-// p. Barriers (LDS traffic only) fall at part boundaries alone.
-template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC>
-__device__ __forceinline__ void fat_parts(const MeshJob &J, int p0, int p1, FatWaveLDS<P, T> &W, const Basis1D &basis,
-                                          const RowBasis<P, T> &rb, double *acc, double *racc, int *lrows, int *lbase,
-                                          int4 *lruns) {
-  using O = Ord<P>;
-  constexpr int THREADS = WAVES * 64, STEP = WAVES * O::EPW, TPT = (MAXR + THREADS - 1) / THREADS;
-  const int tid = threadIdx.x, lane = tid % 64;
-  const int el = lane / O::N, idx = lane % O::N;
-  const bool laneOk = lane < O::LANES;
-  const int myv = (tid / 64) * O::EPW + el;  // this lane's element visit within a batch
-  const Material mat = job_material(J);
-  const int plast = p1 - 1;
-  auto nbat = [&](const PartDesc &d) __attribute__((always_inline)) { return (d.ne + STEP - 1) / STEP; };
-  auto visit = [&](const PartDesc &d, int k) __attribute__((always_inline)) {
-    return d.e0 + min(k * STEP + myv, d.ne - 1);
-  };
-
-  PartDesc dc = part_desc(J, p0);
-  PartDesc dn = part_desc(J, min(p0 + 1, plast));
-  PartDesc dnn = part_desc(J, min(p0 + 2, plast));
-  PartTables tc[TPT], tn[TPT];
-#pragma unroll
-  for (int i = 0; i < TPT; i++) load_part_tables(J, dc, tid + i * THREADS, tc[i]);
-  Gather<P> gc, gn, gnn;
-  gather_level1<P>(J, visit(dc, 0), idx, gc);
-  if (nbat(dc) > 1) gather_level1<P>(J, visit(dc, 1), idx, gn);
-  else gather_level1<P>(J, visit(dn, 0), idx, gn);
-  gather_level2<P>(J, idx, gc);
-
-  int p = p0, k = 0;
-  for (;;) {
-    const int nb = nbat(dc);
-    if (k == 0) {
-      lds_barrier();  // the previous part's write-out has read the accumulators and tables
-#pragma unroll
-      for (int i = 0; i < TPT; i++) {
-        const int t = tid + i * THREADS;
-        if (t < dc.nr) {
-          lrows[t] = tc[i].row;
-          lbase[t] = tc[i].lbase;
-        }
-        if (JAC && t < dc.nruns) lruns[t] = make_int4(tc[i].runLds, tc[i].runGlobal, tc[i].runLen, 0);
-      }
-      if (JAC) zero_planes<THREADS, MAXB>(acc, dc.nb);
-      if (RES) {
-        for (int i = tid; i < dc.nr; i += THREADS) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
-      }
-#pragma unroll
-      for (int i = 0; i < TPT; i++) load_part_tables(J, dn, tid + i * THREADS, tn[i]);
-      lds_barrier();
-    }
-    // batch s+2's position
-    PartDesc d2 = dc;
-    int k2 = k + 2;
-    if (k + 2 >= nb) {
-      if (k + 2 == nb) {
-        d2 = dn;
-        k2 = 0;
-      } else if (nbat(dn) > 1) {
-        d2 = dn;
-        k2 = 1;
-      } else {
-        d2 = dnn;
-        k2 = 0;
-      }
-    }
-    gather_level2<P>(J, idx, gn);
-    gather_level1<P>(J, visit(d2, k2), idx, gnn);
-
-    const bool ok = laneOk && k * STEP + myv < dc.ne;
-    owner_batch_fat<P, T, MAXB, RES, JAC, false>(mat, W, basis, rb, acc, racc, lrows, lbase, dc.nr, gc, ok, el, idx);
-    gc = gn;
-    gn = gnn;
-    if (k + 1 < nb) {
-      k++;
-      continue;
-    }
-    lds_barrier();
-    owner_write<RES, JAC, WAVES, MAXB, false, (P <= 2)>(J, acc, racc, lruns, dc.nruns, lrows, dc.nr);
-    if (p == plast) break;
-    p++;
-    dc = dn;
-    dn = dnn;
-    dnn = part_desc(J, min(p + 2, plast));
-#pragma unroll
-    for (int i = 0; i < TPT; i++) tc[i] = tn[i];
-    k = 0;
-  }
-}
-
-template <int P, typename T, int WAVES, int MAXB, int MAXR, bool RES, bool JAC>
-__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(1, 1))) void owner_fat_kernel(
-    const OwnerArgs args) {
-  using O = Ord<P>;
-  __shared__ FatWaveLDS<P, T> wl[WAVES];
-  __shared__ Basis1D basis;
-  __shared__ __align__(16) double acc[JAC ? 4 * acc_plane<MAXB>() : 2];
-  __shared__ __align__(16) double racc[RES ? 2 * MAXR : 2];
-  __shared__ int lrows[MAXR];
-  __shared__ int lbase[MAXR];
-  __shared__ int4 lruns[JAC ? MAXR : 1];
-
-  // Contiguous range of global parts per workgroup; workgroup w runs on XCD w % 8, and each XCD's
-  // workgroups share one contiguous (spatially adjacent, Morton-ordered) stretch of parts.
-  const int xcd = int(blockIdx.x % 8), slot = int(blockIdx.x / 8), perX = int(gridDim.x / 8);
-  const int numJobs = uniform(args.counts ? args.counts[0] : args.numJobs);
-  const int64_t np = uniform(args.counts ? args.counts[1] : args.totalParts);
-  const int xb = int(np * xcd / 8), xe = int(np * (xcd + 1) / 8);
-  const int pb = xb + int(int64_t(xe - xb) * slot / perX);
-  const int pe = xb + int(int64_t(xe - xb) * (slot + 1) / perX);
-  if (pb >= pe) return;
-
-  copy_basis(basis, args.basis);
-  lds_barrier();
-  RowBasis<P, T> rb;
-  fill_row_basis<P, T>(basis, int(threadIdx.x % 64) % O::N, rb);
-  FatWaveLDS<P, T> &W = wl[threadIdx.x / 64];
-
-  int j = 0;
-  {
-    int lo = 0, hi = numJobs - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) / 2;
-      if (args.jobStart[mid] <= pb) lo = mid;
-      else hi = mid - 1;
-    }
-    j = uniform(lo);
-  }
-  for (int gp = pb; gp < pe; j++) {
-    const int js = uniform(args.jobStart[j]);
-    const int je = uniform(min(args.jobStart[j + 1], pe));
-    const MeshJob J = args.jobs[j];
-    if (je > gp) {
-      fat_parts<P, T, WAVES, MAXB, MAXR, RES, JAC>(J, gp - js, je - js, W, basis, rb, acc, racc, lrows, lbase, lruns);
       gp = je;
     }
   }
@@ -2344,14 +2019,7 @@ struct OwnerLaunch {
 template <int P, typename T, int MAXR, bool RES, bool JAC, bool DET>
 void launch_owner(const OwnerLaunch &L, hipStream_t stream) {
   constexpr OwnerConfig cfg = owner_config(P);
-  if constexpr (!DET && cfg.fat) {
-    if (L.stream) {
-      const int grid = std::min(std::max(8, L.streamGrid / 16 * 8), ((L.args.totalParts + 7) / 8) * 8);
-      owner_fat_kernel<P, T, cfg.waves, cfg.maxBlocks, MAXR, RES, JAC><<<grid, cfg.waves * 64, 0, stream>>>(L.args);
-      return;
-    }
-  }
-  if constexpr (!DET && !cfg.fat && MAXR <= cfg.waves * 64) {
+  if constexpr (!DET && MAXR <= cfg.waves * 64) {
     if (L.stream) {
       // streamGrid is two workgroups per CU; 8-wave workgroups fill a CU's LDS alone.
       const int persistent = cfg.waves >= 8 ? std::max(8, L.streamGrid / 16 * 8) : L.streamGrid;
@@ -2414,8 +2082,7 @@ bool plan_streams(const fem_plan *p) {
   const bool rOnly = (p->flags & FEM_PLAN_RESIDUAL_ONLY) != 0;
   const bool det = (p->flags & FEM_PLAN_DETERMINISTIC) != 0;
   const int epw = 64 / nodes_per_element(p->order);
-  const bool fat = owner_config(p->order).fat != 0;
-  return !rOnly && !det && p->numCUs > 0 && (fat || p->maxPartVisits <= owner_waves(p->order) * epw);
+  return !rOnly && !det && p->numCUs > 0 && p->maxPartVisits <= owner_waves(p->order) * epw;
 }
 
 int stream_grid(int numCUs) { return ((2 * numCUs + 7) / 8) * 8; }

@@ -264,19 +264,10 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
 
   // Greedy cut along that order. A part closes when its rows would exceed the LDS block budget, the
   // row table, or the number of element visits one round of the workgroup's wavefronts covers.
-  // FEM_AMD_PART_VISITS / FEM_AMD_PART_BLOCKS lower these bounds (tuning; never above the kernel's).
   const double scale = std::min(1.0, std::max(partScale, 0.01));
-  int maxVisits = residualOnly ? (1 << 30) : std::max(1, int(cfg.maxVisits * scale));
-  int blockBudget = residualOnly ? maxBlocks : std::max(out.maxRowLength, int(maxBlocks * scale));
+  const int maxVisits = residualOnly ? (1 << 30) : std::max(1, int(cfg.maxVisits * scale));
+  const int blockBudget = residualOnly ? maxBlocks : std::max(out.maxRowLength, int(maxBlocks * scale));
   const int rowBudget = residualOnly ? maxRows : std::max(1, int(maxRows * scale));
-  if (const char *ev = std::getenv("FEM_AMD_PART_VISITS")) {
-    const int v = std::atoi(ev);
-    if (v > 0) maxVisits = v;
-  }
-  if (const char *eb = std::getenv("FEM_AMD_PART_BLOCKS")) {
-    const int v = std::atoi(eb);
-    if (v > 0) blockBudget = std::min(blockBudget, v);
-  }
   std::vector<int> nptr, nadj;
   node_element_adjacency(numNodes, numElements, N, connPtr, conn, nptr, nadj);
   for (int n = 0; n < numNodes; n++) {

@@ -27,45 +27,17 @@ struct OwnerConfig {
   int maxBlocks;
   int maxRows;
   int maxVisits;
-  int fat;  // 1: one workgroup per CU, basis gradients held in registers (several batches per part)
 };
-#ifdef FEM_OWNER_FAT
-// Orders 1-2: one workgroup of 4 waves per CU (one wave per SIMD, up to 512 registers per lane) with
-// the whole 160 KB of LDS and the basis gradients in registers.
-constexpr OwnerConfig owner_config(int order) {
-  return order == 1   ? OwnerConfig{4, 3456, 448, 512, 1}
-         : order == 2 ? OwnerConfig{4, 3584, 320, 224, 1}
-         : order == 3 ? OwnerConfig{4, 1312, 96, 16, 0}
-                      : OwnerConfig{4, 1312, 64, 8, 0};
-}
-#elif defined(FEM_OWNER_WIDE34)
-// All orders: one workgroup of 8 waves per CU sharing an accumulator of 2720 FP64 blocks.
-constexpr OwnerConfig owner_config(int order) {
-  return order == 1   ? OwnerConfig{8, 2720, 384, 128, 0}
-         : order == 2 ? OwnerConfig{8, 2720, 320, 56, 0}
-         : order == 3 ? OwnerConfig{8, 2560, 192, 32, 0}
-                      : OwnerConfig{8, 2560, 128, 16, 0};
-}
-#elif defined(FEM_OWNER_NARROW)
-// Orders 1-2: two workgroups of 4 waves per CU, each with half the LDS.
-constexpr OwnerConfig owner_config(int order) {
-  return order == 1   ? OwnerConfig{4, 1344, 192, 64, 0}
-         : order == 2 ? OwnerConfig{4, 1344, 160, 28, 0}
-         : order == 3 ? OwnerConfig{4, 1312, 96, 16, 0}
-                      : OwnerConfig{4, 1312, 64, 8, 0};
-}
-#else
 // Orders 1-2: one workgroup of 8 waves per CU (two per SIMD) sharing an accumulator of 2720 FP64 blocks,
 // so parts are twice as large as with two 4-wave workgroups and fewer elements are visited twice
 // (1.53 visits per element at 1000x1000 order 2, against 1.83). Orders 3-4: two workgroups of 4 waves
 // per CU, each with half the LDS.
 constexpr OwnerConfig owner_config(int order) {
-  return order == 1   ? OwnerConfig{8, 2720, 352, 128, 0}
-         : order == 2 ? OwnerConfig{8, 2720, 320, 56, 0}
-         : order == 3 ? OwnerConfig{4, 1312, 96, 16, 0}
-                      : OwnerConfig{4, 1312, 64, 8, 0};
+  return order == 1   ? OwnerConfig{8, 2720, 352, 128}
+         : order == 2 ? OwnerConfig{8, 2720, 320, 56}
+         : order == 3 ? OwnerConfig{4, 1312, 96, 16}
+                      : OwnerConfig{4, 1312, 64, 8};
 }
-#endif
 // Residual-only parts: only the per-row residual lives in LDS.
 constexpr int kResidualOnlyMaxRows = 1024;
 

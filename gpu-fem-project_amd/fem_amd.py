@@ -17,8 +17,7 @@ from typing import Optional
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-# FEM_AMD_LIB selects another build of the library (for example a tuning variant).
-LIB_PATH = os.environ.get("FEM_AMD_LIB", os.path.join(HERE, "libfem_amd.so"))
+LIB_PATH = os.path.join(HERE, "libfem_amd.so")
 
 FEM_OK = 0
 FEM_PREC_FP64 = 0
