@@ -42,7 +42,7 @@ POOL_WORKLOADS = {
     "batch256-mixed": "256 independent meshes (parameter seed 6, mesh i seed 6e6+i): order U{1..4}, elements "
                       "logU[1e3, 2e5], square or annulus 50/50, interior jitter +-20%, u,v ~ U(+-1e-3); mixed "
                       "precision, one persistent launch per order/output group, plans autotuned per mesh",
-    "serve-poisson": "serving: Poisson arrivals over a pool of 64 resident meshes (parameter seed 7): order "
+    "serve-poisson": "serving: Poisson arrivals over a pool of resident meshes (64 by default) (parameter seed 7): order "
                      "U{1..4}, elements logU[1e3, 1e6], square or annulus; each request residual-only or "
                      "residual+Jacobian with 50/50 odds; 200 GB HBM budget; mixed-precision served path vs "
                      "plain FP64 per-request launches",
@@ -66,13 +66,15 @@ def make_workload(F, name):
     return mesh, u
 
 
-def plan_kernel_name(args, order):
+def plan_kernel_name(path, order):
     """The kernel(s) a plan assembly launches: owner-computes parts (orders 1-2), or (orders 3-4) one
     element-once traversal that stores exclusive blocks and stages shared ones, then the gathers of the
     staged sums."""
+    if path == "plan-owner" and order >= 3:
+        return "owner_stream_kernel (FP64 row stage; mixed: FP32 MFMA stage)"
     if order >= 3:
         return "element_tiled_kernel+gather_shared"
-    if args.path == "plan-det":
+    if path == "plan-det":
         return "owner_kernel"
     return "owner_stream_kernel"
 
@@ -404,17 +406,24 @@ def bench_serve(args, F, torch):
     warm = np.zeros(2 * pool_size)
     run_served(np.repeat(np.arange(pool_size), 2), np.tile([False, True], pool_size), warm)
 
-    # Saturated throughput: the first S requests all present at t = 0.
+    # Saturated throughput: the first S requests all present at t = 0, three runs of each path.
     sat = min(n_main, 1000)
     zero = np.zeros(sat)
-    _, t_plain_sat = run_plain(mesh_ids[:sat], jacs[:sat], zero)
-    _, t_served_sat = run_served(mesh_ids[:sat], jacs[:sat], zero)
+    t_plain_runs, t_served_runs = [], []
+    for _ in range(3):
+        t_plain_runs.append(run_plain(mesh_ids[:sat], jacs[:sat], zero)[1])
+        t_served_runs.append(run_served(mesh_ids[:sat], jacs[:sat], zero)[1])
+    t_plain_sat, t_served_sat = float(np.median(t_plain_runs)), float(np.median(t_served_runs))
     sat_el = float(elems[mesh_ids[:sat]].sum())
     plain_rate = sat / t_plain_sat
-    log(f"saturated: plain {plain_rate:.1f} req/s, served {sat / t_served_sat:.1f} req/s")
+    log(f"saturated: plain {[round(sat / t, 1) for t in t_plain_runs]} req/s, served "
+        f"{[round(sat / t, 1) for t in t_served_runs]} req/s")
 
     loads = {}
-    for load, n in ((0.8, n_main), (0.5, args.side_requests), (0.95, args.side_requests)):
+    for load in args.serve_loads:
+        n = n_main if load == 0.8 else args.side_requests
+        if n <= 0:
+            continue
         lam = load * plain_rate
         arr = unit_arrivals[:n] / lam
         log(f"load {load}: {n} requests at {lam:.1f} req/s")
@@ -465,11 +474,13 @@ def bench_serve(args, F, torch):
         "vs_baseline": None,
         "dtype": "f32+f64acc",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {POOL_WORKLOADS[args.workload]}", "pool": pool_size,
+        "config": {"workload": f"serve-poisson: {POOL_WORKLOADS['serve-poisson']}", "pool": pool_size,
                    "orders": orders, "elements_min_median_max": [sizes[0], sizes[len(sizes) // 2], sizes[-1]],
                    "slots": 3, "max_batch_jobs": 64, "budget_bytes": 200 * 10 ** 9, "parallelism": "single GPU"},
         "saturated": {"requests": sat, "plain_req_s": plain_rate, "served_req_s": sat / t_served_sat,
-                      "plain_elements_s": sat_el / t_plain_sat, "served_elements_s": sat_el / t_served_sat},
+                      "plain_elements_s": sat_el / t_plain_sat, "served_elements_s": sat_el / t_served_sat,
+                      "plain_req_s_runs": [sat / t for t in t_plain_runs],
+                      "served_req_s_runs": [sat / t for t in t_served_runs]},
         "loads": loads,
         "server": st,
         "cpu_baseline": None,
@@ -486,18 +497,212 @@ def run_pool_workload(args, F, torch):
         bench_serve(args, F, torch)
 
 
+def step_stats(per_s):
+    """Per-step times (seconds) as the reference's harness reports them (verification/TestKernels.cpp:37-60):
+    min / median / average / standard deviation, here in milliseconds."""
+    ms = np.asarray(per_s) * 1e3
+    return {"runs": int(ms.size), "min_ms": float(ms.min()), "median_ms": float(np.median(ms)),
+            "avg_ms": float(ms.mean()), "std_ms": float(ms.std())}
+
+
+def time_events(step, steps, warmup, stream, torch):
+    """Warm-up, then `steps` timed calls bracketed by HIP events on `stream`; returns (wall seconds, per-step
+    event seconds)."""
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        step()
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, np.array([s.elapsed_time(e) for s, e in zip(starts, ends)]) * 1e-3
+
+
+def reference_protocol(step, stream, torch, max_runs, max_seconds, prepare=None):
+    """The reference's timing loop (verification/TestKernels.cpp:29-30, 228-287): one discarded run, then up to
+    max_runs runs or max_seconds, outputs re-initialised outside the timed call (`prepare`)."""
+    if prepare:
+        prepare()
+    step()
+    torch.cuda.synchronize()
+    per = []
+    t_end = time.perf_counter() + max_seconds
+    while len(per) < max_runs and time.perf_counter() < t_end:
+        if prepare:
+            prepare()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        step()
+        b.record(stream)
+        b.synchronize()
+        per.append(a.elapsed_time(b) * 1e-3)
+    return step_stats(per)
+
+
+def run_single(args, F, torch, workload, path, precision, strain="linear", autotune=False, cpu=False,
+               ref_protocol=False, check=True):
+    """One single-mesh assembly leg: builds the workload, times args.steps assemblies (inputs resident in HBM)
+    and returns its record (throughput, per-step statistics, roofline, checks)."""
+    dev = torch.device("cuda:0")
+    prec = F.FEM_PREC_FP64 if precision == "fp64" else F.FEM_PREC_MIXED
+    log(f"leg {workload} / {path} / {precision} / {strain}: building")
+    mesh, u = make_workload(F, workload)
+    pat = F.bsr_pattern(mesh)
+    log(f"mesh: {mesh.num_elements} elements, {mesh.num_nodes} nodes, {pat.num_blocks} blocks")
+    bmap = F.element_bcsr_map(mesh, pat)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    coords, states = d(mesh.coords), d(u)
+    res = torch.zeros(2 * mesh.num_nodes, dtype=torch.float64, device=dev)
+    vals = torch.zeros(4 * pat.num_blocks, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+
+    plan = None
+    kernel_only = None
+    if path.startswith("plan"):
+        t0 = time.time()
+        flags = {"plan-det": F.FEM_PLAN_DETERMINISTIC, "plan-owner": F.FEM_PLAN_OWNER_ONLY}.get(path, 0)
+        if autotune:
+            flags |= F.FEM_PLAN_AUTOTUNE
+        plan = F.Plan(mesh, pat, flags=flags)
+        info = plan.info()
+        log(f"plan built in {time.time() - t0:.1f} s: {info}")
+
+        def step():
+            plan.assemble(states, coords, E_MOD, NU, THICK, res=res, values=vals, precision=prec, stream=stream)
+    else:
+        conn, conn_ptr, dmap = d(mesh.conn), d(mesh.conn_ptr), d(bmap)
+        info = {}
+
+        def kernel():
+            F.jacobian(mesh.order, conn_ptr, conn, mesh.num_elements, states, coords, E_MOD, NU, THICK, dmap, res,
+                       vals, precision=prec, stream=stream, nonlinear=strain == "nonlinear")
+
+        def zero():
+            res.zero_()
+            vals.zero_()
+
+        def step():
+            zero()
+            kernel()
+
+    wall, per = time_events(step, args.steps, args.warmup, stream, torch)
+    ms_per_step = wall / args.steps * 1e3
+    kernel_s = float(per.mean())
+    b_alg = algorithmic_bytes(mesh, pat.num_blocks)
+    achieved = b_alg / kernel_s / 1e9
+    log(f"{ms_per_step:.4f} ms/step wall, kernel avg {kernel_s * 1e3:.4f} ms (min {per.min() * 1e3:.4f}), "
+        f"{achieved:.0f} GB/s algorithmic")
+    protocol = None
+    if ref_protocol:
+        if plan is None:
+            # the reference's number: the kernel alone, outputs zeroed outside the timed call (TestKernels.cpp:237,241)
+            protocol = reference_protocol(kernel, stream, torch, args.ref_runs, args.ref_seconds, prepare=zero)
+            kernel_only = protocol["avg_ms"]
+        else:
+            protocol = reference_protocol(step, stream, torch, args.ref_runs, args.ref_seconds)
+        log(f"reference protocol: {protocol}")
+
+    checks = None
+    if check:
+        # Size-independent properties on the full workload: R = K u, and the plan agrees with the
+        # element-parallel atomic path (a different kernel and summation order). Full-size parity against the
+        # oracle is a test (tests/test_gpu_fullsize.py).
+        rowp_d, cols_d = d(pat.rowp), d(pat.cols)
+        if strain == "nonlinear":
+            # The tangent is the Hessian of the strain energy: symmetric, x^T K z = z^T K x.
+            x = torch.rand_like(res) - 0.5
+            z = torch.rand_like(res) - 0.5
+            kx, kz = torch.zeros_like(res), torch.zeros_like(res)
+            F.bsr_spmv(rowp_d, cols_d, vals, x, kx)
+            F.bsr_spmv(rowp_d, cols_d, vals, z, kz)
+            checks = {"K_symmetry": float((z @ kx - x @ kz).abs() / (x.norm() * kz.norm()))}
+        else:
+            y = torch.zeros_like(res)
+            F.bsr_spmv(rowp_d, cols_d, vals, states, y)
+            checks = {"R_vs_Ku": float((y - res).abs().max() / res.abs().max())}
+        if plan is not None:
+            res2 = torch.zeros_like(res)
+            vals2 = torch.zeros_like(vals)
+            F.jacobian(mesh.order, d(mesh.conn_ptr), d(mesh.conn), mesh.num_elements, states, coords, E_MOD, NU,
+                       THICK, d(bmap), res2, vals2, precision=F.FEM_PREC_FP64)
+            torch.cuda.synchronize()
+            checks["values_vs_fp64_atomic_path"] = float((vals2 - vals).abs().max() / vals2.abs().max())
+            checks["residual_vs_fp64_atomic_path"] = float((res2 - res).abs().max() / res2.abs().max())
+            del res2, vals2
+        log(f"checks: {checks}")
+
+    cpu_rec = None
+    if cpu:
+        log("timing CPU baseline")
+        cpu_rec = cpu_baseline(mesh, u, bmap, pat.num_blocks, args.cpu_seconds)
+        log(f"cpu baseline {cpu_rec['value']:.4g} elements/s on {cpu_rec['cores']} threads")
+
+    traffic = None if autotune else load_traffic(workload, path)
+    if precision != "fp64" or strain != "linear":
+        traffic = None
+    rec = {
+        "workload": workload, "path": path + ("+autotune" if autotune else ""), "precision": precision,
+        "strain": strain, "elements": mesh.num_elements, "nodes": mesh.num_nodes, "bsr_blocks": pat.num_blocks,
+        "elements_per_s": mesh.num_elements * args.steps / wall, "ms_per_step": ms_per_step,
+        "steps": step_stats(per),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": plan_kernel_name(path, mesh.order) if plan is not None
+                     else ("element_kernel_nl" if strain == "nonlinear" else "element_kernel"),
+                     "algorithmic_bytes": b_alg, "kernel_ms": kernel_s * 1e3},
+        "reference_protocol": protocol, "checks": checks, "plan": info, "cpu_baseline": cpu_rec,
+        "dtype": "f64" if prec == F.FEM_PREC_FP64 else "f32+f64acc",
+    }
+    if kernel_only is not None:
+        rec["kernel_only_elements_per_s"] = mesh.num_elements / (kernel_only * 1e-3)
+    if plan is not None:
+        plan.close()
+    del res, vals, coords, states
+    torch.cuda.empty_cache()
+    return rec
+
+
+def serve_leg(args, F, torch):
+    """Compact serving leg of the default run: bench_serve on a smaller pool and request count."""
+    import contextlib
+    import io
+
+    sub = argparse.Namespace(**vars(args))
+    sub.pool, sub.requests, sub.side_requests, sub.serve_loads = args.serve_pool, args.serve_requests, 0, (0.8,)
+    buf = io.StringIO()
+    with contextlib.redirect_stdout(buf):
+        bench_serve(sub, F, torch)
+    rec = json.loads(buf.getvalue().strip().splitlines()[-1])
+    keep = ("config", "saturated", "loads", "server", "checks")
+    return {k: rec[k] for k in keep}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="square1000-p2", choices=sorted(WORKLOADS) + sorted(POOL_WORKLOADS))
-    ap.add_argument("--path", default="plan", choices=["plan", "plan-det", "atomic"])
+    ap.add_argument("--path", default="plan", choices=["plan", "plan-det", "plan-owner", "atomic"],
+                    help="plan-owner: orders 3-4 through the owner-computes parts (FEM_PLAN_OWNER_ONLY)")
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
                     help="default: fp64 for single meshes, mixed for batch256-mixed")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="default workload only: skip the extra legs (mixed, drop-in, annulus, serving)")
+    ap.add_argument("--ref-runs", type=int, default=300, help="reference-protocol runs per leg (the reference: 1000)")
+    ap.add_argument("--ref-seconds", type=float, default=2.0, help="reference-protocol time cap per leg (reference: 30)")
+    ap.add_argument("--serve-pool", type=int, default=16, help="serving leg of the default run: resident meshes")
+    ap.add_argument("--serve-requests", type=int, default=1500, help="serving leg of the default run: requests")
     ap.add_argument("--requests", type=int, default=10000, help="serve-poisson: requests at the main load")
     ap.add_argument("--side-requests", type=int, default=2000, help="serve-poisson: requests at 0.5 and 0.95 load")
     ap.add_argument("--pool", type=int, default=0, help="override the mesh-pool size (0: the workload's own)")
@@ -505,6 +710,7 @@ def main():
     ap.add_argument("--strain", default="linear", choices=["linear", "nonlinear"],
                     help="nonlinear: the Green-strain tangent (drop-in atomic path, FP64)")
     args = ap.parse_args()
+    args.serve_loads = (0.8, 0.5, 0.95)
     if args.autotune and args.path != "plan":
         ap.error("--autotune applies to --path plan only")
     if args.strain == "nonlinear" and (args.path != "atomic" or args.precision == "mixed"):
@@ -524,130 +730,50 @@ def main():
     if args.workload in POOL_WORKLOADS:
         run_pool_workload(args, F, torch)
         return
-    dev = torch.device("cuda:0")
-    prec = F.FEM_PREC_FP64 if args.precision == "fp64" else F.FEM_PREC_MIXED
-    log(f"building workload {args.workload}")
-    mesh, u = make_workload(F, args.workload)
-    pat = F.bsr_pattern(mesh)
-    log(f"mesh: {mesh.num_elements} elements, {mesh.num_nodes} nodes, {pat.num_blocks} blocks")
-    bmap = F.element_bcsr_map(mesh, pat)
-    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
-    coords, states = d(mesh.coords), d(u)
-    res = torch.zeros(2 * mesh.num_nodes, dtype=torch.float64, device=dev)
-    vals = torch.zeros(4 * pat.num_blocks, dtype=torch.float64, device=dev)
-    stream = torch.cuda.current_stream()
-
-    plan = None
-    if args.path.startswith("plan"):
-        t0 = time.time()
-        flags = F.FEM_PLAN_DETERMINISTIC if args.path == "plan-det" else 0
-        if args.autotune:
-            flags |= F.FEM_PLAN_AUTOTUNE
-        plan = F.Plan(mesh, pat, flags=flags)
-        info = plan.info()
-        log(f"plan built in {time.time() - t0:.1f} s: {info}")
-
-        def step():
-            plan.assemble(states, coords, E_MOD, NU, THICK, res=res, values=vals, precision=prec, stream=stream)
-    else:
-        conn, conn_ptr, dmap = d(mesh.conn), d(mesh.conn_ptr), d(bmap)
-        info = {}
-
-        def step():
-            res.zero_()
-            vals.zero_()
-            F.jacobian(mesh.order, conn_ptr, conn, mesh.num_elements, states, coords, E_MOD, NU, THICK, dmap, res,
-                       vals, precision=prec, stream=stream, nonlinear=args.strain == "nonlinear")
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    log(f"timing {args.steps} steps")
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record(stream)
-        step()
-        ends[i].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    per = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)]) * 1e-3
-    ms_per_step = wall / args.steps * 1e3
-    value = mesh.num_elements * args.steps / wall
-    kernel_s = float(per.mean())
-    b_alg = algorithmic_bytes(mesh, pat.num_blocks)
-    achieved = b_alg / kernel_s / 1e9
-    log(f"{ms_per_step:.4f} ms/step wall, kernel avg {kernel_s * 1e3:.4f} ms (min {per.min() * 1e3:.4f}), "
-        f"{achieved:.0f} GB/s algorithmic")
-
-    check = None
-    if not args.no_check:
-        # Size-independent properties on the full workload: R = K u, and the plan agrees with the
-        # element-parallel atomic path (a different kernel and summation order).
-        rowp_d, cols_d = d(pat.rowp), d(pat.cols)
-        if args.strain == "nonlinear":
-            # The tangent is the Hessian of the strain energy: symmetric, x^T K z = z^T K x.
-            x = torch.rand_like(res) - 0.5
-            z = torch.rand_like(res) - 0.5
-            kx, kz = torch.zeros_like(res), torch.zeros_like(res)
-            F.bsr_spmv(rowp_d, cols_d, vals, x, kx)
-            F.bsr_spmv(rowp_d, cols_d, vals, z, kz)
-            sym = float((z @ kx - x @ kz).abs() / (x.norm() * kz.norm()))
-            check = {"K_symmetry": sym}
-        else:
-            y = torch.zeros_like(res)
-            F.bsr_spmv(rowp_d, cols_d, vals, states, y)
-            r_ku = float((y - res).abs().max() / res.abs().max())
-            check = {"R_vs_Ku": r_ku}
-        if plan is not None:
-            res2 = torch.zeros_like(res)
-            vals2 = torch.zeros_like(vals)
-            F.jacobian(mesh.order, d(mesh.conn_ptr), d(mesh.conn), mesh.num_elements, states, coords, E_MOD, NU,
-                       THICK, d(bmap), res2, vals2, precision=F.FEM_PREC_FP64)
-            torch.cuda.synchronize()
-            check["values_vs_atomic_path"] = float((vals2 - vals).abs().max() / vals2.abs().max())
-            check["residual_vs_atomic_path"] = float((res2 - res).abs().max() / res2.abs().max())
-            del res2, vals2
-        log(f"checks: {check}")
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("timing CPU baseline")
-        cpu = cpu_baseline(mesh, u, bmap, pat.num_blocks, args.cpu_seconds)
-        log(f"cpu baseline {cpu['value']:.4g} elements/s on {cpu['cores']} threads")
-
-    traffic = None if args.autotune else load_traffic(args.workload, args.path)
+    default_run = (args.workload == "square1000-p2" and args.path == "plan" and args.precision == "fp64"
+                   and args.strain == "linear" and not args.autotune and not args.no_legs)
+    main_rec = run_single(args, F, torch, args.workload, args.path, args.precision, strain=args.strain,
+                          autotune=args.autotune, cpu=not args.no_cpu_baseline, ref_protocol=True,
+                          check=not args.no_check)
+    legs = None
+    if default_run:
+        # The rest of the metric "res+Jac elements/s & % HBM roofline; served p50/p99 latency vs plain":
+        # the 1M-element mesh in mixed precision and through the reference's entry point (drop-in atomic), the
+        # order-4 annulus in both precisions, and a compact serving run against plain per-request launches.
+        legs = {}
+        for wl, path, prec in (("square1000-p2", "plan", "mixed"), ("square1000-p2", "atomic", "fp64"),
+                               ("annulus200x1250-p4", "plan", "fp64"), ("annulus200x1250-p4", "plan", "mixed")):
+            r = run_single(args, F, torch, wl, path, prec, ref_protocol=True, check=not args.no_check)
+            legs[f"{wl}/{path}/{prec}"] = r
+        log("serving leg")
+        legs["serving"] = serve_leg(args, F, torch)
+    r = main_rec
     out = {
         "metric": "res+Jac elements/s & % HBM roofline",
-        "value": value,
+        "value": r["elements_per_s"],
         "unit": "elements/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == F.FEM_PREC_FP64 else "f32+f64acc",
+        "dtype": r["dtype"],
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]['desc']}",
-                   "elements": mesh.num_elements, "nodes": mesh.num_nodes, "bsr_blocks": pat.num_blocks,
-                   "path": args.path + ("+autotune" if args.autotune else ""), "precision": args.precision,
-                   "strain": args.strain, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": plan_kernel_name(args, mesh.order) if plan is not None
-                     else ("element_kernel_nl" if args.strain == "nonlinear" else "element_kernel"),
-                     "algorithmic_bytes": b_alg, "kernel_ms": kernel_s * 1e3},
-        "cpu_baseline": cpu,
-        "checks": check,
-        "plan": info,
+                   "elements": r["elements"], "nodes": r["nodes"], "bsr_blocks": r["bsr_blocks"],
+                   "path": r["path"], "precision": args.precision, "strain": args.strain,
+                   "parallelism": "single GPU"},
+        "roofline": r["roofline"],
+        "cpu_baseline": r["cpu_baseline"],
+        "step_stats": r["steps"],
+        "reference_protocol": r["reference_protocol"],
+        "checks": r["checks"],
+        "plan": r["plan"],
+        "legs": legs,
     }
     print(json.dumps(out), flush=True)
-    if plan is not None:
-        plan.close()
 
 
 if __name__ == "__main__":

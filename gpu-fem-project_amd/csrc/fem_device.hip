@@ -2358,8 +2358,8 @@ int create_plan(int order, int numNodes, int numElements, const int *connPtr, co
   // Orders 3-4: Jacobian assemblies traverse the elements once (the owner-computes parts visit each element
   // 3-4 times there). Most blocks of an element's K_e are formed by that element alone and are stored
   // directly; the rest are staged and summed by a gather pass (SplitHost). Without room for int codes the
-  // plan keeps the owner-computes path.
-  if (order >= 3 && !(flags & FEM_PLAN_RESIDUAL_ONLY)) {
+  // plan keeps the owner-computes path (as FEM_PLAN_OWNER_ONLY plans do).
+  if (order >= 3 && !(flags & (FEM_PLAN_RESIDUAL_ONLY | FEM_PLAN_OWNER_ONLY))) {
     const int N = (order + 1) * (order + 1);
     std::vector<int> c(size_t(numElements) * N);
     for (int e = 0; e < numElements; e++) std::copy(conn + connPtr[e], conn + connPtr[e] + N, c.begin() + size_t(e) * N);
@@ -2750,6 +2750,22 @@ int fem_batch_clear(fem_batch *batch) {
 
 int fem_batch_add(fem_batch *batch, fem_plan *plan, const double *states, const double *nodeCoords, double E,
                   double nu, double t, double *residual, double *matEntries) {
+  return fem::batch_add_staged(batch, plan, states, nodeCoords, E, nu, t, residual, matEntries, nullptr);
+}
+
+}  // extern "C"
+
+namespace fem {
+
+int64_t batch_stage_doubles(const fem_plan *plan, bool withResidual) {
+  if (!plan || !plan->dConn) return 0;
+  return 4 * plan->numSlots + (withResidual ? 2 * plan->numRslots : 0);
+}
+
+bool plan_streams_public(const fem_plan *plan) { return plan && plan_streams(plan); }
+
+int batch_add_staged(fem_batch *batch, fem_plan *plan, const double *states, const double *nodeCoords, double E,
+                     double nu, double t, double *residual, double *matEntries, double *stage) {
   if (!batch || !plan) return set_error(FEM_ERR_NULL_POINTER, "batch/plan is NULL");
   if (!states || !nodeCoords) return set_error(FEM_ERR_NULL_POINTER, "states/nodeCoords is NULL");
   if (!residual && !matEntries) return set_error(FEM_ERR_NULL_POINTER, "residual and matEntries are both NULL");
@@ -2778,10 +2794,18 @@ int fem_batch_add(fem_batch *batch, fem_plan *plan, const double *states, const 
     g.unitStart.push_back(int(units));
     g.gStart.push_back(int(staged));
     g.rStart.push_back(int(rstaged));
-    g.stageOff.push_back(batch->splitDoubles);
-    batch->splitDoubles += 4 * plan->numSlots;
-    g.rstageOff.push_back(batch->splitDoubles);
-    if (residual) batch->splitDoubles += 2 * plan->numRslots;
+    if (stage) {
+      // caller-owned staging: fixed now, never in the batch's arena
+      g.jobs.back().stage = stage;
+      g.jobs.back().rstage = stage + 4 * plan->numSlots;
+      g.stageOff.push_back(-1);
+      g.rstageOff.push_back(-1);
+    } else {
+      g.stageOff.push_back(batch->splitDoubles);
+      batch->splitDoubles += 4 * plan->numSlots;
+      g.rstageOff.push_back(batch->splitDoubles);
+      if (residual) batch->splitDoubles += 2 * plan->numRslots;
+    }
     batch->numJobs++;
     return FEM_OK;
   }
@@ -2799,6 +2823,10 @@ int fem_batch_add(fem_batch *batch, fem_plan *plan, const double *states, const 
   batch->numJobs++;
   return FEM_OK;
 }
+
+}  // namespace fem
+
+extern "C" {
 
 int fem_batch_assemble(fem_batch *batch, int precision, void *stream, double *elapsedSeconds) {
   if (!batch) return set_error(FEM_ERR_NULL_POINTER, "batch is NULL");
@@ -2838,6 +2866,7 @@ int fem_batch_assemble(fem_batch *batch, int precision, void *stream, double *el
   }
   for (auto &g : batch->split) {
     for (size_t j = 0; j < g.jobs.size(); j++) {
+      if (g.stageOff[j] < 0) continue;  // caller-owned staging
       g.jobs[j].stage = batch->dArena + g.stageOff[j];
       g.jobs[j].rstage = batch->dArena + g.rstageOff[j];
     }

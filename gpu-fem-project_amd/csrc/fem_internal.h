@@ -94,6 +94,14 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
                     const int *rowp, const int *cols, const double *nodeCoords, int flags, PlanHost &out,
                     double partScale = 1.0);
 
+// Batches (fem_device.hip). Doubles of element-once staging a Jacobian job of `plan` takes (0: the plan has no
+// element-once tables); fem_batch_add with that staging owned by the caller (`stage`, NULL: the batch's own
+// arena, grown at assembly); whether a plan's parts fit the streaming kernel (graph batches need it).
+int64_t batch_stage_doubles(const fem_plan *plan, bool withResidual);
+int batch_add_staged(fem_batch *batch, fem_plan *plan, const double *states, const double *nodeCoords, double E,
+                     double nu, double t, double *residual, double *matEntries, double *stage);
+bool plan_streams_public(const fem_plan *plan);
+
 }  // namespace fem
 
 // Opaque plan handle (device-resident arrays built from a PlanHost).

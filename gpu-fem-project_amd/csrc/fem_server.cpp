@@ -8,6 +8,11 @@
 // replayed through a captured hipGraph) on one of several streams ("slots"): requests that arrive while
 // earlier batches run are collected into the next batch, which starts on a free slot and overlaps the
 // running ones on the device.
+//
+// Every device byte the server uses while serving comes from the budget: the resident meshes, and one arena
+// (sized at the first admission to what the budget leaves after them) holding each admitted request's outputs
+// and the element-once staging of its batch job. Staging is returned when the batch completes, outputs when the
+// client releases the request. Serving makes no allocator call.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -46,10 +51,13 @@ enum RequestState { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3, kReleased
 struct Request {
   int64_t id = 0;
   int mesh = 0;
+  ServerMesh m;  // copy of the mesh record, taken at submission under the lock (the table may grow meanwhile)
   const double *states = nullptr;
   bool withJacobian = false;
   double *residual = nullptr;
   double *values = nullptr;
+  double *stage = nullptr;  // element-once staging of the batch job (arena; returned when the batch completes)
+  int64_t stageBytes = 0;
   int64_t bytes = 0;
   RequestState state = kQueued;
   int status = FEM_OK;
@@ -77,7 +85,8 @@ struct fem_server {
   // hipFree would synchronise the device and stall the batches in flight).
   char *arena = nullptr;
   int64_t arenaBytes = 0;
-  int64_t outputBytes = 0;                 // bytes of the arena held by admitted requests
+  int64_t outputBytes = 0;                 // bytes of the arena held by admitted requests (outputs + staging)
+  int64_t peakOutputBytes = 0;             // high-water mark of outputBytes
   std::map<int64_t, int64_t> freeRanges;   // arena offset -> length, coalesced
   std::unordered_map<int64_t, std::unique_ptr<Request>> requests;
   std::deque<Request *> queue;
@@ -101,6 +110,18 @@ int64_t output_bytes(const ServerMesh &m, bool withJacobian) {
 constexpr int64_t kArenaAlign = 256;
 
 int64_t align_up(int64_t n) { return (n + kArenaAlign - 1) / kArenaAlign * kArenaAlign; }
+
+// Element-once staging of a request's batch job, in bytes (0: none).
+int64_t stage_bytes(const ServerMesh &m, bool withJacobian) {
+  return withJacobian ? fem::batch_stage_doubles(m.plan, true) * int64_t(sizeof(double)) : 0;
+}
+
+// Arena bytes a request holds while its batch runs (aligned as pool_alloc aligns them).
+int64_t arena_need(const ServerMesh &m, bool withJacobian) {
+  const int64_t st = stage_bytes(m, withJacobian);
+  return align_up(int64_t(m.numNodes) * 16) + (withJacobian ? align_up(m.numBlocks * 32) : 0) +
+         (st ? align_up(st) : 0);
+}
 
 // Creates the output arena with the budget left after the resident meshes. Caller holds mu.
 bool ensure_arena(fem_server *s) {
@@ -127,6 +148,7 @@ void *pool_alloc(fem_server *s, size_t bytes) {
     s->freeRanges.erase(it);
     if (len > need) s->freeRanges.emplace(off + need, len - need);
     s->outputBytes += need;
+    s->peakOutputBytes = std::max(s->peakOutputBytes, s->outputBytes);
     return s->arena + off;
   }
   return nullptr;
@@ -157,14 +179,15 @@ void admit_into(fem_server *s, std::vector<Request *> &out, int capacity) {
   int perGroup[12] = {0};
   while (!s->queue.empty() && int(out.size()) < capacity) {
     Request *r = s->queue.front();
-    const ServerMesh &m = s->meshes[size_t(r->mesh)];
+    const ServerMesh &m = r->m;
     const int g = (m.order - 1) * 3 + (r->withJacobian ? 2 : 0);
     if (perGroup[g] >= s->cfg.maxBatchJobs) break;
     const size_t rb = size_t(m.numNodes) * 2 * sizeof(double);
     const size_t vb = r->withJacobian ? size_t(m.numBlocks) * 4 * sizeof(double) : 0;
+    const size_t sb = size_t(stage_bytes(m, r->withJacobian));
     double *res = static_cast<double *>(pool_alloc(s, rb));
     if (!res) break;
-    double *vals = nullptr;
+    double *vals = nullptr, *stage = nullptr;
     if (vb) {
       vals = static_cast<double *>(pool_alloc(s, vb));
       if (!vals) {
@@ -172,30 +195,59 @@ void admit_into(fem_server *s, std::vector<Request *> &out, int capacity) {
         break;
       }
     }
+    if (sb) {
+      stage = static_cast<double *>(pool_alloc(s, sb));
+      if (!stage) {
+        pool_free(s, vals, vb);
+        pool_free(s, res, rb);
+        break;
+      }
+    }
     r->residual = res;
     r->values = vals;
+    r->stage = stage;
+    r->stageBytes = int64_t(sb);
     s->queue.pop_front();
     perGroup[g]++;
     out.push_back(r);
   }
 }
 
+// Marks a request finished (status FEM_OK: done, else failed) and returns its staging to the arena. Caller holds mu.
+void finish_request(fem_server *s, Request *r, double t, int status, const char *err) {
+  if (r->stage) {
+    pool_free(s, r->stage, size_t(r->stageBytes));
+    r->stage = nullptr;
+    r->stageBytes = 0;
+  }
+  r->tDone = t;
+  r->status = status;
+  if (status != FEM_OK) {
+    r->state = kFailed;
+    r->error = err ? err : "";
+  } else {
+    r->state = kDone;
+  }
+  s->completed++;
+}
+
 void complete_slot(fem_server *s, Slot &slot, int status, const char *err) {
   const double t = now_seconds();
   std::lock_guard<std::mutex> lk(s->mu);
-  for (Request *r : slot.requests) {
-    r->tDone = t;
-    r->status = status;
-    if (status != FEM_OK) {
-      r->state = kFailed;
-      r->error = err ? err : "";
-    } else {
-      r->state = kDone;
-    }
-    s->completed++;
-  }
+  for (Request *r : slot.requests) finish_request(s, r, t, status, err);
   slot.requests.clear();
   slot.busy = false;
+  s->doneCv.notify_all();
+  s->cv.notify_one();  // staging returned: queued requests may now fit
+}
+
+// Fails every request still queued (the server is being destroyed). Caller holds mu.
+void fail_queued(fem_server *s) {
+  const double t = now_seconds();
+  while (!s->queue.empty()) {
+    finish_request(s, s->queue.front(), t, FEM_ERR_INVALID_ARGUMENT, "the server was destroyed before admission");
+    s->queue.pop_front();
+  }
   s->doneCv.notify_all();
 }
 
@@ -219,7 +271,12 @@ void worker_loop(fem_server *s) {
       std::vector<Request *> take;
       {
         std::lock_guard<std::mutex> lk(s->mu);
-        if (s->stop && s->queue.empty()) break;
+        if (s->stop) {
+          // Destroying: requests not yet admitted fail (they may wait for memory that only their client's
+          // release would free); the batches in flight drain below.
+          fail_queued(s);
+          break;
+        }
         admit_into(s, take, 12 * s->cfg.maxBatchJobs);
         const double t = now_seconds();
         for (Request *r : take) {
@@ -229,15 +286,25 @@ void worker_loop(fem_server *s) {
       }
       if (take.empty()) break;
       int rc = fem_batch_clear(slot.batch);
+      std::vector<Request *> added;
       for (Request *r : take) {
-        if (rc) break;
-        const ServerMesh &m = s->meshes[size_t(r->mesh)];
-        rc = fem_batch_add(slot.batch, m.plan, r->states, m.coords, m.E, m.nu, m.t, r->residual,
-                           r->withJacobian ? r->values : nullptr);
+        const ServerMesh &m = r->m;
+        const int arc = rc ? rc
+                           : fem::batch_add_staged(slot.batch, m.plan, r->states, m.coords, m.E, m.nu, m.t, r->residual,
+                                              r->withJacobian ? r->values : nullptr, r->stage);
+        if (arc) {
+          // only this request fails; the rest of the batch runs
+          std::lock_guard<std::mutex> lk(s->mu);
+          finish_request(s, r, now_seconds(), arc, fem_last_error());
+          s->doneCv.notify_all();
+        } else {
+          added.push_back(r);
+        }
       }
-      if (!rc) rc = fem_batch_assemble(slot.batch, s->cfg.precision, slot.stream, nullptr);
+      if (added.empty()) continue;
+      rc = fem_batch_assemble(slot.batch, s->cfg.precision, slot.stream, nullptr);
       if (!rc && hipEventRecord(slot.done, slot.stream) != hipSuccess) rc = FEM_ERR_HIP;
-      slot.requests = take;
+      slot.requests = added;
       slot.busy = true;
       if (rc) {
         complete_slot(s, slot, rc, fem_last_error());
@@ -319,6 +386,13 @@ int fem_server_add_mesh(fem_server *server, int order, int numNodes, int numElem
   rc = fem_plan_create(order, numNodes, numElements, connPtr, conn, rowp.data(), cols.data(), nodeCoordsHost, 0,
                        &m.plan);
   if (rc) return rc;
+  if (server->cfg.useGraphs && !fem::plan_streams_public(m.plan)) {
+    // A graph batch replays the streaming kernel, whose parts are one batch per wavefront; this mesh's parts
+    // are larger (a node shared by more elements than a workgroup takes at once).
+    fem_plan_destroy(m.plan);
+    return set_error(FEM_ERR_UNSUPPORTED, "the mesh's parts do not fit the graph-replayed batch kernel; "
+                     "create the server with useGraphs = 0 to serve it");
+  }
   int64_t planBytes = 0;
   fem_plan_info(m.plan, nullptr, nullptr, nullptr, &planBytes);
   m.residentBytes = planBytes + int64_t(numNodes) * 16;
@@ -365,13 +439,14 @@ int fem_server_submit(fem_server *server, int meshId, const double *states, int 
   const ServerMesh &m = server->meshes[size_t(meshId)];
   const int64_t need = output_bytes(m, withJacobian != 0);
   const int64_t room = server->arena ? server->arenaBytes : server->cfg.hbmBudgetBytes - server->residentBytes;
-  if (align_up(int64_t(m.numNodes) * 16) + (withJacobian ? align_up(m.numBlocks * 32) : 0) > room) {
-    return set_error(FEM_ERR_BUDGET, "request outputs need %lld bytes; the budget leaves %lld after resident meshes",
-                     (long long)need, (long long)room);
+  if (arena_need(m, withJacobian != 0) > room) {
+    return set_error(FEM_ERR_BUDGET, "request outputs and staging need %lld bytes; the budget leaves %lld after "
+                     "resident meshes", (long long)arena_need(m, withJacobian != 0), (long long)room);
   }
   auto r = std::make_unique<Request>();
   r->id = server->nextId++;
   r->mesh = meshId;
+  r->m = m;
   r->states = states;
   r->withJacobian = withJacobian != 0;
   r->bytes = need;
@@ -417,12 +492,11 @@ int fem_server_read(fem_server *server, int64_t requestId, double *residualHost,
   int rc = fem_server_wait(server, requestId, &res, &vals, nullptr);
   if (rc) return rc;
   int64_t nn = 0, nb = 0;
-  int mesh = 0;
   {
     std::lock_guard<std::mutex> lk(server->mu);
-    mesh = server->requests[requestId]->mesh;
-    nn = server->meshes[size_t(mesh)].numNodes;
-    nb = server->meshes[size_t(mesh)].numBlocks;
+    const Request &r = *server->requests[requestId];
+    nn = r.m.numNodes;
+    nb = r.m.numBlocks;
   }
   if (residualHost && hipMemcpy(residualHost, res, size_t(nn) * 16, hipMemcpyDeviceToHost) != hipSuccess) {
     return set_error(FEM_ERR_HIP, "copying the residual failed");
@@ -442,8 +516,8 @@ int fem_server_release(fem_server *server, int64_t requestId) {
   if (r->state != kDone && r->state != kFailed) {
     return set_error(FEM_ERR_INVALID_ARGUMENT, "request %lld has not completed", (long long)requestId);
   }
-  const ServerMesh &m = server->meshes[size_t(r->mesh)];
-  pool_free(server, r->residual, size_t(m.numNodes) * 16);
+  const ServerMesh &m = r->m;
+  if (r->residual) pool_free(server, r->residual, size_t(m.numNodes) * 16);
   if (r->values) pool_free(server, r->values, size_t(m.numBlocks) * 32);
   server->requests.erase(it);
   server->cv.notify_one();
@@ -456,6 +530,17 @@ int fem_server_stats(fem_server *server, int64_t *batches, int64_t *completed, i
   if (batches) *batches = server->batches;
   if (completed) *completed = server->completed;
   if (bytesInUse) *bytesInUse = server->residentBytes + server->outputBytes;  // arena bytes held by requests
+  return FEM_OK;
+}
+
+int fem_server_memory(fem_server *server, int64_t *residentBytes, int64_t *arenaBytes, int64_t *arenaInUse,
+                      int64_t *arenaPeak) {
+  if (!server) return set_error(FEM_ERR_NULL_POINTER, "server is NULL");
+  std::lock_guard<std::mutex> lk(server->mu);
+  if (residentBytes) *residentBytes = server->residentBytes;
+  if (arenaBytes) *arenaBytes = server->arena ? server->arenaBytes : 0;
+  if (arenaInUse) *arenaInUse = server->outputBytes;
+  if (arenaPeak) *arenaPeak = server->peakOutputBytes;
   return FEM_OK;
 }
 

@@ -25,6 +25,7 @@ FEM_PREC_MIXED = 1
 FEM_PLAN_DETERMINISTIC = 1
 FEM_PLAN_RESIDUAL_ONLY = 2
 FEM_PLAN_AUTOTUNE = 4
+FEM_PLAN_OWNER_ONLY = 8
 FEM_BATCH_GRAPH = 1
 
 STATUS_NAMES = {
@@ -95,6 +96,7 @@ SIGNATURES = {
     "fem_server_read": ([_P, _I64, _P, _P], _I),
     "fem_server_release": ([_P, _I64], _I),
     "fem_server_stats": ([_P, _PI64, _PI64, _PI64], _I),
+    "fem_server_memory": ([_P, _PI64, _PI64, _PI64, _PI64], _I),
     "fem_server_destroy": ([_P], _I),
 }
 
@@ -535,6 +537,12 @@ class Server:
         v = [ctypes.c_int64() for _ in range(3)]
         _check(self._lib.fem_server_stats(self.handle, *[ctypes.byref(x) for x in v]))
         return dict(zip(["batches", "completed", "bytes_in_use"], [x.value for x in v]))
+
+    def memory(self):
+        """Device bytes of the server: resident meshes, the arena, arena bytes held now and their peak."""
+        v = [ctypes.c_int64() for _ in range(4)]
+        _check(self._lib.fem_server_memory(self.handle, *[ctypes.byref(x) for x in v]))
+        return dict(zip(["resident", "arena", "arena_in_use", "arena_peak"], [x.value for x in v]))
 
     def close(self):
         if self.handle:

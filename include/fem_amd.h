@@ -195,6 +195,9 @@ int fem_plan_create(int order, int numNodes, int numElements, const int *connPtr
                                     mixed-precision residual + Jacobian assembly of each on the device (zero
                                     states, temporary outputs of the mesh's size) and keep the fastest; smaller
                                     parts give small meshes more workgroups. Not with the flags above. */
+#define FEM_PLAN_OWNER_ONLY 8    /* orders 3-4: no element-once tables; Jacobians take the owner-computes parts
+                                    (FP64: row-restricted VALU Gram stage; mixed: FP32 MFMA Gram stage), the
+                                    route plans take when the element-once codes would overflow an int */
 
 /* Part-size scale a plan was built with (1, 0.5 or 0.25; 1 without FEM_PLAN_AUTOTUNE) and, for an autotuned
  * plan, the measured seconds of the chosen build (0 otherwise). No counterpart in the reference: the
@@ -233,14 +236,17 @@ int fem_batch_destroy(fem_batch *batch);
 
 /* ---- serving loop ------------------------------------------------------------------------------
  * Resident meshes and continuous batching. A worker thread admits queued requests in FIFO order
- * while their outputs fit the device-memory budget (resident meshes + live outputs), packs them into
- * batches (fem_batch_*) and runs each batch on one of numSlots streams, so batches formed while
- * others run overlap them on the device. Outputs are allocated by the server (pooled) when a request
- * is admitted and held until fem_server_release. Request states are caller device memory that must be
- * complete when submitted and stay valid until the request completes. Times are steady-clock seconds. */
+ * while their outputs and batch staging fit the device-memory budget (resident meshes + live outputs +
+ * staging of the batches in flight), packs them into batches (fem_batch_*) and runs each batch on one of
+ * numSlots streams, so batches formed while others run overlap them on the device. Outputs are carved
+ * from the server's arena when a request is admitted and held until fem_server_release; staging is
+ * returned when its batch completes. A request whose batch job is rejected fails alone. Request states are
+ * caller device memory that must be complete when submitted and stay valid until the request completes.
+ * fem_server_destroy fails the requests still queued and waits for the batches in flight. Times are
+ * steady-clock seconds. */
 typedef struct fem_server fem_server;
 typedef struct fem_server_config {
-  int64_t hbmBudgetBytes; /* resident meshes + request outputs */
+  int64_t hbmBudgetBytes; /* resident meshes + request outputs + batch staging */
   int maxBatchJobs;       /* jobs per (order, outputs) group in one batch */
   int numSlots;           /* batches in flight (streams), 1..8 */
   int precision;          /* FEM_PREC_FP64 or FEM_PREC_MIXED */
@@ -249,7 +255,8 @@ typedef struct fem_server_config {
 
 int fem_server_create(const fem_server_config *cfg, fem_server **server);
 /* Builds the mesh's BSR pattern and resident plan (host arrays in, as fem_plan_create) and uploads its
- * coordinates. FEM_ERR_BUDGET if it does not fit. */
+ * coordinates. FEM_ERR_BUDGET if it does not fit; FEM_ERR_UNSUPPORTED (with useGraphs) if its parts do not
+ * fit the graph-replayed kernel. */
 int fem_server_add_mesh(fem_server *server, int order, int numNodes, int numElements, const int *connPtr,
                         const int *conn, const double *nodeCoordsHost, double E, double nu, double t, int *meshId);
 int fem_server_mesh_info(fem_server *server, int meshId, int64_t *numNodes, int64_t *numBlocks,
@@ -266,6 +273,12 @@ int fem_server_read(fem_server *server, int64_t requestId, double *residualHost,
 /* Returns the request's outputs to the pool. */
 int fem_server_release(fem_server *server, int64_t requestId);
 int fem_server_stats(fem_server *server, int64_t *batches, int64_t *completed, int64_t *bytesInUse);
+/* Device memory of the server, all within hbmBudgetBytes: resident meshes (plans + coordinates), the arena
+ * (created at the first admission with the budget left after the resident meshes; request outputs and the
+ * element-once staging of their batch jobs are carved from it, so serving makes no allocator call), the arena
+ * bytes held now and their high-water mark. Any pointer may be NULL. */
+int fem_server_memory(fem_server *server, int64_t *residentBytes, int64_t *arenaBytes, int64_t *arenaInUse,
+                      int64_t *arenaPeak);
 int fem_server_destroy(fem_server *server);
 
 /* ---- BSR utilities (device) -------------------------------------------------------------------

@@ -205,3 +205,35 @@ def test_batches_sharing_plans_run_concurrently(fem, torch_gpu, meshes):
         b.close()
     for plan in plans:
         plan.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_staging_regrows(fem, torch_gpu, meshes, graph):
+    """A batch's own element-once staging arena grows when a later composition needs more: one order 3-4 job,
+    then every order 3-4 job three times over (several times the staging). Results against the oracle after
+    the regrow, on replays with eager work in between."""
+    torch = torch_gpu
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")  # noqa: E731
+    high = [c for c in meshes if c["mesh"].order >= 3]
+    plans = [fem.Plan(c["mesh"], c["pat"]) for c in high]
+    batch = fem.Batch(max_jobs=16, graph=graph)
+    for rep, compo in enumerate(([0], list(range(len(high))) * 3)):
+        batch.clear()
+        outs = []
+        for k in compo:
+            c = high[k]
+            m = c["mesh"]
+            res = torch.full((2 * m.num_nodes,), float("nan"), dtype=torch.float64, device="cuda:0")
+            vals = torch.full((4 * c["pat"].num_blocks,), float("nan"), dtype=torch.float64, device="cuda:0")
+            batch.add(plans[k], d(c["u"]), d(m.coords), E, NU, T, res=res, values=vals)
+            outs.append((c, res, vals))
+        for _ in range(2):
+            batch.assemble()
+            torch.cuda.synchronize()
+            _ = torch.ones(1024, device="cuda:0").sum().item()  # eager work between replays
+            for c, res, vals in outs:
+                assert textbook.normwise_rel(res.cpu().numpy(), c["res"]) < 1e-12
+                assert textbook.normwise_rel(vals.cpu().numpy(), c["vals"]) < 1e-12
+    batch.close()
+    for p in plans:
+        p.close()

@@ -237,3 +237,29 @@ def test_autotuned_plan_matches_oracle(fem, orc, torch_gpu, order):
     assert textbook.normwise_rel(res.cpu().numpy(), ref_res) < 1e-12
     assert textbook.normwise_rel(vals.cpu().numpy(), ref_vals) < 1e-12
     plan.close()
+
+
+@pytest.mark.parametrize("case_id", [c for c in CASES if c[1] >= 3], ids=[f"{k}-p{p}" for k, p in CASES if p >= 3])
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("det", [False, True], ids=["atomicLDS", "det"])
+def test_owner_route_orders_3_4(fem, torch_gpu, cases, case_id, precision, det):
+    """Orders 3-4 through the owner-computes parts (FEM_PLAN_OWNER_ONLY: the route a plan takes when its
+    element-once codes would overflow an int): FP64 row-restricted Gram stage, mixed FP32 MFMA Gram stage.
+    NaN-prefilled outputs, against the oracle."""
+    c = cases[case_id]
+    m = c["mesh"]
+    D = Dev(torch_gpu, c)
+    tol = FP64_TOL if precision == 0 else MIXED_TOL
+    flags = fem.FEM_PLAN_OWNER_ONLY | (fem.FEM_PLAN_DETERMINISTIC if det else 0)
+    plan = fem.Plan(m, c["pat"], flags=flags)
+    res = D.nan(D.n_res)
+    vals = D.nan(D.n_val)
+    plan.assemble(D.states, D.coords, E, NU, T, res=res, values=vals, precision=precision)
+    r, v = host(res, torch_gpu), host(vals, torch_gpu)
+    assert np.isfinite(r).all() and np.isfinite(v).all()
+    assert textbook.normwise_rel(v, c["vals"]) < tol
+    assert textbook.normwise_rel(r, c["res"]) < tol
+    vals_only = D.nan(D.n_val)
+    plan.assemble(D.states, D.coords, E, NU, T, values=vals_only, precision=precision)
+    assert textbook.normwise_rel(host(vals_only, torch_gpu), c["vals"]) < tol
+    plan.close()

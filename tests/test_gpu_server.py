@@ -3,7 +3,9 @@
 Checks: each served result against the oracle (FP64 server: 1e-12; mixed-precision server: 1e-5);
 a request gives the same result alone and inside a burst of other requests (1e-14 normwise: sums of a
 part use LDS atomics, so equality is to rounding); requests whose outputs exceed the budget wait until
-earlier results are released; oversize meshes and requests are rejected with FEM_ERR_BUDGET.
+earlier results are released; oversize meshes and requests are rejected with FEM_ERR_BUDGET; resident
+meshes + outputs + staging stay inside the budget during a burst; destroying a server with queued and
+unreleased requests returns; meshes added while requests run.
 """
 
 import numpy as np
@@ -170,3 +172,84 @@ def test_graph_batches_replay(fem, torch_gpu, pool):
     batch.close()
     for p in plans:
         p.close()
+
+
+def test_memory_within_budget_during_burst(fem, torch_gpu, pool):
+    """Resident meshes + outputs + element-once staging stay inside the budget: the server's own accounting, and
+    the device's free memory seen from outside, while a burst (orders 1-4, Jacobians with staging) is served."""
+    torch = torch_gpu
+    dstates = [[_dev(torch, u) for u in c["states"]] for c in pool]
+    torch.cuda.synchronize()
+    free0, _ = torch.cuda.mem_get_info()
+    budget = 1 << 30
+    srv = fem.Server(budget_bytes=budget, max_batch_jobs=8, slots=3, precision=0, graphs=True)
+    ids = [srv.add_mesh(c["mesh"], E, NU, T) for c in pool]
+    free_min = torch.cuda.mem_get_info()[0]
+    rids = []
+    for k in range(40):
+        i = k % len(pool)
+        rids.append((srv.submit(ids[i], dstates[i][k % 3], True), i, k % 3))
+    for rid, i, rep in rids:
+        free_min = min(free_min, torch.cuda.mem_get_info()[0])
+        mem = srv.memory()
+        assert mem["resident"] + mem["arena"] <= budget
+        assert mem["arena_in_use"] <= mem["arena"] and mem["arena_peak"] <= mem["arena"]
+        c = pool[i]
+        res, vals = srv.read(rid, c["mesh"].num_nodes, c["pat"].num_blocks)
+        assert textbook.normwise_rel(vals, c["refs"][rep][1]) < 1e-12
+        assert textbook.normwise_rel(res, c["refs"][rep][0]) < 1e-12
+        srv.release(rid)
+    mem = srv.memory()
+    assert mem["arena_in_use"] == 0 and mem["arena_peak"] > 0
+    used = free0 - free_min
+    # the budget, plus the batches' job tables and graphs (a few MB per slot)
+    assert used <= budget + (64 << 20), (used, budget)
+    srv.close()
+
+
+def test_destroy_returns_with_queued_and_unreleased_requests(fem, torch_gpu, pool):
+    """Destroying a server whose queued requests wait for memory held by unreleased results returns: the queued
+    requests fail and the batches in flight drain."""
+    import threading
+
+    c = pool[1]
+    m = c["mesh"]
+    probe = fem.Server(budget_bytes=1 << 40, max_batch_jobs=4, slots=1, precision=1)
+    info = probe.mesh_info(probe.add_mesh(m, E, NU, T))
+    probe.close()
+    out_bytes = 16 * info["nodes"] + 32 * info["blocks"]
+    srv = fem.Server(budget_bytes=info["resident_bytes"] + 2 * out_bytes + out_bytes // 2, max_batch_jobs=4,
+                     slots=2, precision=1)
+    mid = srv.add_mesh(m, E, NU, T)
+    u = _dev(torch_gpu, c["states"][0])
+    torch_gpu.cuda.synchronize()
+    rids = [srv.submit(mid, u, True) for _ in range(5)]
+    srv.wait(rids[0])
+    srv.wait(rids[1])
+    assert not srv.poll(rids[2])  # waits for memory that only a release would free
+    done = threading.Event()
+    th = threading.Thread(target=lambda: (srv.close(), done.set()), daemon=True)
+    th.start()
+    assert done.wait(30), "fem_server_destroy did not return"
+
+
+def test_add_mesh_while_serving(fem, orc, torch_gpu, pool):
+    """Meshes added while a burst is in flight: every result still matches the oracle."""
+    torch = torch_gpu
+    srv = fem.Server(budget_bytes=4 << 30, max_batch_jobs=8, slots=3, precision=0, graphs=True)
+    ids = {0: srv.add_mesh(pool[0]["mesh"], E, NU, T)}
+    dstates = [[_dev(torch, u) for u in c["states"]] for c in pool]
+    torch.cuda.synchronize()
+    rids = []
+    for k in range(1, len(pool)):
+        for _ in range(6):
+            rids.append((srv.submit(ids[0], dstates[0][k % 3], True), 0, k % 3))
+        ids[k] = srv.add_mesh(pool[k]["mesh"], E, NU, T)  # the mesh table grows while requests run
+        rids.append((srv.submit(ids[k], dstates[k][0], True), k, 0))
+    for rid, i, rep in rids:
+        c = pool[i]
+        res, vals = srv.read(rid, c["mesh"].num_nodes, c["pat"].num_blocks)
+        assert textbook.normwise_rel(vals, c["refs"][rep][1]) < 1e-12
+        assert textbook.normwise_rel(res, c["refs"][rep][0]) < 1e-12
+        srv.release(rid)
+    srv.close()

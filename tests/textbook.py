@@ -88,6 +88,23 @@ def assemble_dense(p: int, coords: np.ndarray, conn: np.ndarray, E: float, nu: f
     return K
 
 
+def assemble_bsr(p: int, coords: np.ndarray, conn: np.ndarray, rowp: np.ndarray, cols: np.ndarray, E: float,
+                 nu: float, t: float) -> np.ndarray:
+    """BSR values (2x2 row-major blocks) of the global stiffness, each element's K_e added block by block at the
+    slot found by a search of the sorted columns of its row (for meshes too large for the dense form)."""
+    vals = np.zeros(len(cols) * 4)
+    for e in range(conn.shape[0]):
+        nodes = conn[e]
+        Ke = element_stiffness(p, coords[nodes], E, nu, t)
+        for i, r in enumerate(nodes):
+            row = cols[rowp[r]:rowp[r + 1]]
+            slots = rowp[r] + np.searchsorted(row, nodes)
+            assert np.array_equal(cols[slots], nodes)
+            for j, k in enumerate(slots):
+                vals[4 * k: 4 * k + 4] += Ke[2 * i: 2 * i + 2, 2 * j: 2 * j + 2].reshape(-1)
+    return vals
+
+
 def dense_to_bsr_values(K: np.ndarray, rowp: np.ndarray, cols: np.ndarray) -> np.ndarray:
     """Extract 2x2 blocks of a dense matrix in BSR (row-major block) order."""
     vals = np.zeros(len(cols) * 4)
