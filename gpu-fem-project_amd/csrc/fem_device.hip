@@ -2694,6 +2694,15 @@ int capture_batch_graph(fem_batch *b, int precision) {
 
 }  // namespace
 
+int fem::batch_prepare_graph(fem_batch *b, int precision) {
+  if (!b) return set_error(FEM_ERR_NULL_POINTER, "batch is NULL");
+  if (precision != FEM_PREC_FP64 && precision != FEM_PREC_MIXED) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "unknown precision mode %d", precision);
+  }
+  if (!(b->flags & FEM_BATCH_GRAPH) || b->graph[precision]) return FEM_OK;
+  return capture_batch_graph(b, precision);
+}
+
 extern "C" {
 
 int fem_batch_create(int maxJobs, int flags, fem_batch **batch) {

@@ -101,6 +101,9 @@ int64_t batch_stage_doubles(const fem_plan *plan, bool withResidual);
 int batch_add_staged(fem_batch *batch, fem_plan *plan, const double *states, const double *nodeCoords, double E,
                      double nu, double t, double *residual, double *matEntries, double *stage);
 bool plan_streams_public(const fem_plan *plan);
+// Records a graph batch's launch graph for `precision` now (it does not depend on the batch's contents), so that
+// no capture is in flight later, when other threads may be using the device. No-op without FEM_BATCH_GRAPH.
+int batch_prepare_graph(fem_batch *batch, int precision);
 
 }  // namespace fem
 

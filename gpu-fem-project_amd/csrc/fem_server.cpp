@@ -123,10 +123,9 @@ int64_t arena_need(const ServerMesh &m, bool withJacobian) {
          (st ? align_up(st) : 0);
 }
 
-// Creates the output arena with the budget left after the resident meshes. Caller holds mu.
-bool ensure_arena(fem_server *s) {
-  if (s->arena) return true;
-  const int64_t bytes = (s->cfg.hbmBudgetBytes - s->residentBytes) / kArenaAlign * kArenaAlign;
+// Creates the output arena of `bytes` bytes. Caller holds mu.
+bool create_arena(fem_server *s, int64_t bytes) {
+  bytes = bytes / kArenaAlign * kArenaAlign;
   if (bytes <= 0) return false;
   if (hipMalloc(reinterpret_cast<void **>(&s->arena), size_t(bytes)) != hipSuccess) {
     s->arena = nullptr;
@@ -136,6 +135,13 @@ bool ensure_arena(fem_server *s) {
   s->freeRanges.clear();
   s->freeRanges.emplace(0, bytes);
   return true;
+}
+
+// Creates the output arena, unless fem_server_reserve has, with the budget left after the resident meshes.
+// Caller holds mu.
+bool ensure_arena(fem_server *s) {
+  if (s->arena) return true;
+  return create_arena(s, s->cfg.hbmBudgetBytes - s->residentBytes);
 }
 
 // First-fit allocation from the arena. Caller holds mu.
@@ -355,7 +361,10 @@ int fem_server_create(const fem_server_config *cfg, fem_server **server) {
   for (Slot &slot : s->slots) {
     if (hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&slot.done, hipEventDisableTiming) != hipSuccess ||
-        fem_batch_create(cfg->maxBatchJobs, cfg->useGraphs ? FEM_BATCH_GRAPH : 0, &slot.batch) != FEM_OK) {
+        fem_batch_create(cfg->maxBatchJobs, cfg->useGraphs ? FEM_BATCH_GRAPH : 0, &slot.batch) != FEM_OK ||
+        fem::batch_prepare_graph(slot.batch, cfg->precision) != FEM_OK) {
+      // (graphs are recorded here, before the worker starts: a capture in flight would make other threads'
+      // default-stream work, such as fem_server_add_mesh's uploads, fail)
       fem_server_destroy(s);
       return set_error(FEM_ERR_HIP, "creating server streams/events/batches failed");
     }
@@ -530,6 +539,19 @@ int fem_server_stats(fem_server *server, int64_t *batches, int64_t *completed, i
   if (batches) *batches = server->batches;
   if (completed) *completed = server->completed;
   if (bytesInUse) *bytesInUse = server->residentBytes + server->outputBytes;  // arena bytes held by requests
+  return FEM_OK;
+}
+
+int fem_server_reserve(fem_server *server, int64_t arenaBytes) {
+  if (!server) return set_error(FEM_ERR_NULL_POINTER, "server is NULL");
+  std::lock_guard<std::mutex> lk(server->mu);
+  if (server->arena) return set_error(FEM_ERR_INVALID_ARGUMENT, "the output arena exists already");
+  if (arenaBytes < kArenaAlign) return set_error(FEM_ERR_INVALID_ARGUMENT, "arena of %lld bytes", (long long)arenaBytes);
+  if (server->residentBytes + arenaBytes > server->cfg.hbmBudgetBytes) {
+    return set_error(FEM_ERR_BUDGET, "an arena of %lld bytes exceeds the budget left after resident meshes (%lld)",
+                     (long long)arenaBytes, (long long)(server->cfg.hbmBudgetBytes - server->residentBytes));
+  }
+  if (!create_arena(server, arenaBytes)) return set_error(FEM_ERR_OUT_OF_MEMORY, "hipMalloc of the arena failed");
   return FEM_OK;
 }
 

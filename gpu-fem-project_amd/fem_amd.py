@@ -65,6 +65,9 @@ SIGNATURES = {
     "fem_split_tables": ([_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _PI64, _PI64, _PI64, _PI64], _I),
     "fem_bsr_pattern_device": ([_I, _I, _I, _P, _P, _P, _P, _I64, _PI64, _P], _I),
     "fem_write_array": ([_P, _I64, ctypes.c_char_p], _I),
+    "fem_read_bdf_size": ([ctypes.c_char_p, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
+    "fem_read_bdf": ([ctypes.c_char_p, _I, _P, _P, _P, _P], _I),
+    "fem_write_bdf": ([ctypes.c_char_p, _I, _I, _I, _P, _P, _P], _I),
     "fem_write_bsr_matrix": ([_I, _I, _I, _P, _P, _P, _I, _I, ctypes.c_char_p], _I),
     "fem_element_bcsr_map_device": ([_P, _P, _I, _I, _I, _P, _P, _I, _P, _P], _I),
     "fem_element_colouring": ([_I, _I, _I, _P, _P, _P, ctypes.POINTER(_I)], _I),
@@ -97,6 +100,7 @@ SIGNATURES = {
     "fem_server_release": ([_P, _I64], _I),
     "fem_server_stats": ([_P, _PI64, _PI64, _PI64], _I),
     "fem_server_memory": ([_P, _PI64, _PI64, _PI64, _PI64], _I),
+    "fem_server_reserve": ([_P, _I64], _I),
     "fem_server_destroy": ([_P], _I),
 }
 
@@ -309,6 +313,33 @@ def write_bsr_matrix(pattern: "Pattern", values: np.ndarray, path: str, bsize: i
                                        _ptr(pattern.cols), _ptr(v), max_rows, max_cols, os.fsencode(path)))
 
 
+FEM_BDF_NASTRAN_ORDER = 1
+
+
+def read_bdf(path: str, nastran_order: bool = False):
+    """Mesh from a BDF file (GRID and CQUADn cards): returns (Mesh, GRID ids of the nodes)."""
+    lib = load()
+    flags = FEM_BDF_NASTRAN_ORDER if nastran_order else 0
+    order, nn, ne = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(lib.fem_read_bdf_size(path.encode(), flags, ctypes.byref(order), ctypes.byref(nn), ctypes.byref(ne)))
+    npe = (order.value + 1) ** 2
+    coords = np.zeros((nn.value, 2))
+    conn_ptr = np.zeros(ne.value + 1, dtype=np.int32)
+    conn = np.zeros((ne.value, npe), dtype=np.int32)
+    ids = np.zeros(nn.value, dtype=np.int64)
+    _check(lib.fem_read_bdf(path.encode(), flags, _ptr(coords), _ptr(conn_ptr), _ptr(conn), _ptr(ids)))
+    return Mesh(order=order.value, coords=coords, conn=conn), ids
+
+
+def write_bdf(path: str, mesh) -> None:
+    """Writes the mesh as GRID (free field, full precision) and CQUADn cards (tensor-product node order)."""
+    cp = mesh.conn_ptr
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    coords = np.ascontiguousarray(mesh.coords, dtype=np.float64)
+    _check(load().fem_write_bdf(path.encode(), mesh.order, mesh.num_nodes, mesh.num_elements, _ptr(cp), _ptr(conn),
+                                _ptr(coords)))
+
+
 def bsr_pattern_device(num_nodes, num_elements, npe, conn_ptr, conn, stream=None):
     """Device-built pattern from device arrays (torch int32 tensors): returns (rowp, cols) torch tensors on the
     same device, identical to bsr_pattern's arrays."""
@@ -317,11 +348,12 @@ def bsr_pattern_device(num_nodes, num_elements, npe, conn_ptr, conn, stream=None
     rowp = torch.empty(num_nodes + 1, dtype=torch.int32, device=conn.device)
     nb = ctypes.c_int64(0)
     lib = load()
-    _check(lib.fem_bsr_pattern_device(num_nodes, num_elements, npe, _dptr(conn_ptr), _dptr(conn), _dptr(rowp), None,
-                                      0, ctypes.byref(nb), _stream(stream)))
+    fn = lib.fem_bsr_pattern_device
+    _check(fn(num_nodes, num_elements, npe, _dptr(conn_ptr), _dptr(conn), _dptr(rowp), None, 0, ctypes.byref(nb),
+              _stream(stream)))
     cols = torch.empty(max(nb.value, 1), dtype=torch.int32, device=conn.device)
-    _check(lib.fem_bsr_pattern_device(num_nodes, num_elements, npe, _dptr(conn_ptr), _dptr(conn), _dptr(rowp),
-                                      _dptr(cols), cols.numel(), ctypes.byref(nb), _stream(stream)))
+    _check(fn(num_nodes, num_elements, npe, _dptr(conn_ptr), _dptr(conn), _dptr(rowp), _dptr(cols), cols.numel(),
+              ctypes.byref(nb), _stream(stream)))
     return rowp, cols[: nb.value]
 
 
@@ -537,6 +569,10 @@ class Server:
         v = [ctypes.c_int64() for _ in range(3)]
         _check(self._lib.fem_server_stats(self.handle, *[ctypes.byref(x) for x in v]))
         return dict(zip(["batches", "completed", "bytes_in_use"], [x.value for x in v]))
+
+    def reserve(self, arena_bytes: int):
+        """Creates the output arena now with arena_bytes (the rest of the budget stays open to meshes)."""
+        _check(self._lib.fem_server_reserve(self.handle, int(arena_bytes)))
 
     def memory(self):
         """Device bytes of the server: resident meshes, the arena, arena bytes held now and their peak."""

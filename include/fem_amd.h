@@ -122,6 +122,24 @@ int fem_write_array(const double *values, int64_t n, const char *path);
 int fem_write_bsr_matrix(int nrows, int ncols, int bsize, const int *rowp, const int *cols, const double *values,
                          int maxRows, int maxCols, const char *path);
 
+/* ---- BDF mesh files (host) ------------------------------------------------------------------------
+ * The reference reads its meshes from Nastran bulk-data files through TACS (TACSMeshLoader::scanBDFFile,
+ * verification/TACSHelpers.h:86-114; element by card: CQUAD4, CQUAD/CQUAD9, CQUAD16, CQUAD25), written by gmsh
+ * and rewritten by fixGmshBDF (meshes/GmshUtils.py:118-196). fem_read_bdf reads GRID (small, large GRID* and
+ * free field) and CQUADn cards with continuation lines; nodes are numbered 0..n-1 in GRID-card order (their ids
+ * go to gridIds, optional), 2-D coordinates X1, X2 (CP and X3 must be 0). Element nodes are taken in card
+ * order, or with FEM_BDF_NASTRAN_ORDER converted from Nastran's CQUAD4 / CQUAD8-9 order (corners
+ * counter-clockwise, mid-sides, centre) to the tensor-product order a = ay (p+1) + ax. One element order per
+ * file. fem_read_bdf_size gives the order and counts (parse once for the sizes, once for the arrays:
+ * coords [numNodes*2], connPtr [numElements+1], conn [numElements*N]). fem_write_bdf writes GRID in free field
+ * with "%.17g" reals (read back bit-exact) and CQUADn in fixed small field, nine fields per line, as
+ * fixGmshBDF lays them out; nodes and elements numbered from 1. */
+#define FEM_BDF_NASTRAN_ORDER 1
+int fem_read_bdf_size(const char *path, int flags, int *order, int *numNodes, int *numElements);
+int fem_read_bdf(const char *path, int flags, double *coords, int *connPtr, int *conn, long long *gridIds);
+int fem_write_bdf(const char *path, int order, int numNodes, int numElements, const int *connPtr, const int *conn,
+                  const double *nodeCoords);
+
 /* ---- BSR pattern and element map on the device ---------------------------------------------------
  * The same pattern and map as fem_bsr_pattern_rowp/_cols and fem_element_bcsr_map (identical arrays), built
  * on the GPU from device arrays, in work ordered on `stream` (a hipStream_t; NULL = default stream). These
@@ -279,6 +297,10 @@ int fem_server_stats(fem_server *server, int64_t *batches, int64_t *completed, i
  * bytes held now and their high-water mark. Any pointer may be NULL. */
 int fem_server_memory(fem_server *server, int64_t *residentBytes, int64_t *arenaBytes, int64_t *arenaInUse,
                       int64_t *arenaPeak);
+/* Creates the arena now with arenaBytes (before the first admission; otherwise the first admission gives it the
+ * whole budget left), so the rest of the budget stays open to meshes added while requests are served.
+ * FEM_ERR_BUDGET if it does not fit next to the resident meshes. */
+int fem_server_reserve(fem_server *server, int64_t arenaBytes);
 int fem_server_destroy(fem_server *server);
 
 /* ---- BSR utilities (device) -------------------------------------------------------------------

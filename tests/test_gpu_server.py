@@ -237,6 +237,7 @@ def test_add_mesh_while_serving(fem, orc, torch_gpu, pool):
     """Meshes added while a burst is in flight: every result still matches the oracle."""
     torch = torch_gpu
     srv = fem.Server(budget_bytes=4 << 30, max_batch_jobs=8, slots=3, precision=0, graphs=True)
+    srv.reserve(1 << 30)  # the other 3 GB stay open to meshes added while requests run
     ids = {0: srv.add_mesh(pool[0]["mesh"], E, NU, T)}
     dstates = [[_dev(torch, u) for u in c["states"]] for c in pool]
     torch.cuda.synchronize()
