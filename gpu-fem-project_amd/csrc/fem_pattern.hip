@@ -16,6 +16,8 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
+#include <vector>
 
 #include "fem_amd.h"
 #include "fem_internal.h"
@@ -231,6 +233,11 @@ __global__ __launch_bounds__(kThreads) void element_map(int numRows, int numElem
   map[k] = pos * bs2;
 }
 
+__global__ __launch_bounds__(kThreads) void fill_zero(int *p, int64_t n) {
+  const int64_t k = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (k < n) p[k] = 0;
+}
+
 // Scratch owned by one call; freed on every return path.
 struct Scratch {
   void *p[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -242,18 +249,20 @@ struct Scratch {
 
 unsigned grid_for(int64_t n) { return unsigned((n + kThreads - 1) / kThreads); }
 
-// Exclusive scan of in[0, n) into out[0, n] (see the scan kernels).
-template <typename OutT>
-int exclusive_scan(const int *in, int64_t n, OutT *out, int *flags, hipStream_t stream) {
+// Block sums an exclusive scan of n entries (n + 1 outputs) needs.
+int64_t scan_blocks(int64_t n) {
   const int64_t perBlock = int64_t(kThreads) * kScanItems;
-  const int64_t blocks = (n + 1 + perBlock - 1) / perBlock;  // n + 1 outputs
-  int64_t *sums = nullptr;
-  FEM_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&sums), size_t(blocks) * sizeof(int64_t), stream));
+  return (n + 1 + perBlock - 1) / perBlock;
+}
+
+// Exclusive scan of in[0, n) into out[0, n] (see the scan kernels), with scan_blocks(n) block sums in `sums`.
+template <typename OutT>
+int exclusive_scan(const int *in, int64_t n, OutT *out, int *flags, int64_t *sums, hipStream_t stream) {
+  const int64_t blocks = scan_blocks(n);
   scan_block_sums<<<unsigned(blocks), kThreads, 0, stream>>>(in, n, sums);
   scan_sums<<<1, kThreads, 0, stream>>>(sums, blocks);
   scan_apply<OutT><<<unsigned(blocks), kThreads, 0, stream>>>(in, n, sums, out, flags);
   FEM_HIP_CHECK(hipGetLastError());
-  FEM_HIP_CHECK(hipFreeAsync(sums, stream));
   return FEM_OK;
 }
 
@@ -276,9 +285,36 @@ int check_flags(const int *dflags, hipStream_t stream) {
 
 using namespace fem;
 
-extern "C" int fem_bsr_pattern_device(int numNodes, int numElements, int nodesPerElement, const int *connPtr,
-                                      const int *conn, int *rowp, int *cols, int64_t colsCapacity, int64_t *numBlocks,
-                                      void *streamPtr) {
+// Scratch of fem_bsr_pattern_device_noalloc: one array per GPU, held by the process (64 MiB). Host code never
+// uses the array's name as an address: each device's copy is reached through the address
+// hipGetSymbolAddress gives on that device, looked up once per device (device_scratch).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace fem {
+namespace {
+
+// Scratch of one pattern build: error flags, per-node reference counts (then the fill cursor), node->element
+// offsets, the adjacency list, row lengths and the scans' block sums.
+struct PatternScratch {
+  int *flags, *cnt, *nptr, *adj, *len;
+  int64_t *sums;
+};
+
+constexpr size_t kScratchAlign = 256;
+size_t align_bytes(size_t n) { return (n + kScratchAlign - 1) / kScratchAlign * kScratchAlign; }
+
+// Bytes of each scratch array of a build over numNodes nodes and refs node references, in PatternScratch order.
+void pattern_scratch_sizes(int numNodes, int64_t refs, size_t (&bytes)[6]) {
+  const size_t nodes1 = (size_t(numNodes) + 1) * sizeof(int);
+  bytes[0] = 4 * sizeof(int);
+  bytes[1] = bytes[2] = bytes[4] = nodes1;
+  bytes[3] = size_t(std::max<int64_t>(refs, 1)) * sizeof(int);
+  bytes[5] = size_t(scan_blocks(numNodes)) * sizeof(int64_t);
+}
+
+int check_pattern_args(int numNodes, int numElements, int nodesPerElement, const int *connPtr, const int *conn,
+                       const int *rowp, const int64_t *numBlocks) {
   if (numNodes < 0 || numElements < 0 || nodesPerElement <= 0) {
     return set_error(FEM_ERR_INVALID_ARGUMENT, "negative size (numNodes=%d numElements=%d nodesPerElement=%d)",
                      numNodes, numElements, nodesPerElement);
@@ -288,27 +324,22 @@ extern "C" int fem_bsr_pattern_device(int numNodes, int numElements, int nodesPe
   if (int64_t(numElements) * nodesPerElement > INT32_MAX) {
     return set_error(FEM_ERR_UNSUPPORTED, "more than 2^31 element node references");
   }
-  hipStream_t stream = static_cast<hipStream_t>(streamPtr);
-  Scratch sc;
-  int *flags, *cnt, *nptr, *adj, *len;
+  return FEM_OK;
+}
+
+// The pattern build of fem_bsr_pattern_device on caller-provided scratch (arguments already checked).
+int pattern_build(int numNodes, int numElements, int nodesPerElement, const int *connPtr, const int *conn, int *rowp,
+                  int *cols, int64_t colsCapacity, int64_t *numBlocks, hipStream_t stream, const PatternScratch &sc) {
   const int64_t refs = int64_t(numElements) * nodesPerElement;
-  FEM_HIP_CHECK(hipMalloc(&sc.p[0], 4 * sizeof(int)));
-  FEM_HIP_CHECK(hipMalloc(&sc.p[1], (size_t(numNodes) + 1) * sizeof(int)));
-  FEM_HIP_CHECK(hipMalloc(&sc.p[2], (size_t(numNodes) + 1) * sizeof(int)));
-  FEM_HIP_CHECK(hipMalloc(&sc.p[3], size_t(std::max<int64_t>(refs, 1)) * sizeof(int)));
-  FEM_HIP_CHECK(hipMalloc(&sc.p[4], (size_t(numNodes) + 1) * sizeof(int)));
-  flags = static_cast<int *>(sc.p[0]);
-  cnt = static_cast<int *>(sc.p[1]);
-  nptr = static_cast<int *>(sc.p[2]);
-  adj = static_cast<int *>(sc.p[3]);
-  len = static_cast<int *>(sc.p[4]);
-  FEM_HIP_CHECK(hipMemsetAsync(flags, 0, 4 * sizeof(int), stream));
-  FEM_HIP_CHECK(hipMemsetAsync(cnt, 0, (size_t(numNodes) + 1) * sizeof(int), stream));
+  int *flags = sc.flags, *cnt = sc.cnt, *nptr = sc.nptr, *adj = sc.adj, *len = sc.len;
+  // cleared with a fill kernel (scratch may be reused across calls; no memset nodes)
+  fill_zero<<<1, kThreads, 0, stream>>>(flags, 4);
+  fill_zero<<<grid_for(int64_t(numNodes) + 1), kThreads, 0, stream>>>(cnt, int64_t(numNodes) + 1);
   if (refs > 0) {
     count_refs<<<grid_for(refs), kThreads, 0, stream>>>(numNodes, numElements, nodesPerElement, connPtr, conn, cnt,
                                                         flags);
   }
-  int rc = exclusive_scan<int>(cnt, numNodes, nptr, flags, stream);
+  int rc = exclusive_scan<int>(cnt, numNodes, nptr, flags, sc.sums, stream);
   if (rc) return rc;
   rc = check_flags(flags, stream);  // invalid ids must not reach the fill
   if (rc) return rc;
@@ -323,7 +354,7 @@ extern "C" int fem_bsr_pattern_device(int numNodes, int numElements, int nodesPe
     row_columns<false><<<rowBlocks, kThreads, 0, stream>>>(numNodes, nodesPerElement, connPtr, conn, nptr, adj, len,
                                                           nullptr, nullptr, flags);
   }
-  rc = exclusive_scan<int>(len, numNodes, rowp, flags, stream);
+  rc = exclusive_scan<int>(len, numNodes, rowp, flags, sc.sums, stream);
   if (rc) return rc;
   FEM_HIP_CHECK(hipGetLastError());
   int total = 0;
@@ -343,6 +374,75 @@ extern "C" int fem_bsr_pattern_device(int numNodes, int numElements, int nodesPe
     if (rc) return rc;
   }
   return FEM_OK;
+}
+
+std::mutex scratchMu;
+std::vector<void *> scratchAddr;  // per device ordinal; NULL until looked up
+
+int device_scratch(void **out) {
+  int dev = 0;
+  FEM_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(scratchMu);
+  if (dev >= int(scratchAddr.size())) scratchAddr.resize(size_t(dev) + 1, nullptr);
+  if (!scratchAddr[size_t(dev)]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      return set_error(FEM_ERR_HIP, "hipGetSymbolAddress of the pattern scratch failed on device %d: %s", dev,
+                       hipGetErrorString(e));
+    }
+    scratchAddr[size_t(dev)] = p;
+  }
+  *out = scratchAddr[size_t(dev)];
+  return FEM_OK;
+}
+
+}  // namespace
+}  // namespace fem
+
+extern "C" int fem_bsr_pattern_device(int numNodes, int numElements, int nodesPerElement, const int *connPtr,
+                                      const int *conn, int *rowp, int *cols, int64_t colsCapacity, int64_t *numBlocks,
+                                      void *streamPtr) {
+  int rc = check_pattern_args(numNodes, numElements, nodesPerElement, connPtr, conn, rowp, numBlocks);
+  if (rc) return rc;
+  hipStream_t stream = static_cast<hipStream_t>(streamPtr);
+  size_t bytes[6];
+  pattern_scratch_sizes(numNodes, int64_t(numElements) * nodesPerElement, bytes);
+  Scratch sc;
+  for (int i = 0; i < 6; i++) FEM_HIP_CHECK(hipMalloc(&sc.p[i], bytes[i]));
+  const PatternScratch ps{static_cast<int *>(sc.p[0]), static_cast<int *>(sc.p[1]), static_cast<int *>(sc.p[2]),
+                          static_cast<int *>(sc.p[3]), static_cast<int *>(sc.p[4]), static_cast<int64_t *>(sc.p[5])};
+  return pattern_build(numNodes, numElements, nodesPerElement, connPtr, conn, rowp, cols, colsCapacity, numBlocks,
+                       stream, ps);
+}
+
+extern "C" int fem_bsr_pattern_device_noalloc(int numNodes, int numElements, int nodesPerElement, const int *connPtr,
+                                              const int *conn, int *rowp, int *cols, int64_t colsCapacity,
+                                              int64_t *numBlocks, void *streamPtr) {
+  int rc = check_pattern_args(numNodes, numElements, nodesPerElement, connPtr, conn, rowp, numBlocks);
+  if (rc) return rc;
+  size_t bytes[6];
+  pattern_scratch_sizes(numNodes, int64_t(numElements) * nodesPerElement, bytes);
+  size_t need = 0;
+  for (size_t b : bytes) need += align_bytes(b);
+  if (need > kNoAllocScratchBytes) {
+    return set_error(FEM_ERR_INVALID_ARGUMENT, "the pattern of %d nodes and %lld node references needs %zu bytes of "
+                     "scratch; fem_bsr_pattern_device_noalloc holds %zu (use fem_bsr_pattern_device)", numNodes,
+                     (long long)(int64_t(numElements) * nodesPerElement), need, kNoAllocScratchBytes);
+  }
+  void *base = nullptr;
+  rc = device_scratch(&base);
+  if (rc) return rc;
+  unsigned char *p = static_cast<unsigned char *>(base);
+  void *arr[6];
+  for (int i = 0; i < 6; i++) {
+    arr[i] = p;
+    p += align_bytes(bytes[i]);
+  }
+  const PatternScratch ps{static_cast<int *>(arr[0]), static_cast<int *>(arr[1]), static_cast<int *>(arr[2]),
+                          static_cast<int *>(arr[3]), static_cast<int *>(arr[4]), static_cast<int64_t *>(arr[5])};
+  return pattern_build(numNodes, numElements, nodesPerElement, connPtr, conn, rowp, cols, colsCapacity, numBlocks,
+                       static_cast<hipStream_t>(streamPtr), ps);
 }
 
 extern "C" int fem_element_bcsr_map_device(const int *connPtr, const int *conn, int numElements, int nodesPerElement,

@@ -64,6 +64,7 @@ SIGNATURES = {
     "fem_element_bcsr_map": ([_P, _P, _I, _I, _I, _P, _P, _I, _P], _I),
     "fem_split_tables": ([_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _PI64, _PI64, _PI64, _PI64], _I),
     "fem_bsr_pattern_device": ([_I, _I, _I, _P, _P, _P, _P, _I64, _PI64, _P], _I),
+    "fem_bsr_pattern_device_noalloc": ([_I, _I, _I, _P, _P, _P, _P, _I64, _PI64, _P], _I),
     "fem_write_array": ([_P, _I64, ctypes.c_char_p], _I),
     "fem_read_bdf_size": ([ctypes.c_char_p, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "fem_read_bdf": ([ctypes.c_char_p, _I, _P, _P, _P, _P], _I),
@@ -340,15 +341,16 @@ def write_bdf(path: str, mesh) -> None:
                                 _ptr(coords)))
 
 
-def bsr_pattern_device(num_nodes, num_elements, npe, conn_ptr, conn, stream=None):
+def bsr_pattern_device(num_nodes, num_elements, npe, conn_ptr, conn, stream=None, noalloc=False):
     """Device-built pattern from device arrays (torch int32 tensors): returns (rowp, cols) torch tensors on the
-    same device, identical to bsr_pattern's arrays."""
+    same device, identical to bsr_pattern's arrays. noalloc: through fem_bsr_pattern_device_noalloc (the
+    library's fixed per-GPU scratch instead of allocations)."""
     import torch
 
     rowp = torch.empty(num_nodes + 1, dtype=torch.int32, device=conn.device)
     nb = ctypes.c_int64(0)
     lib = load()
-    fn = lib.fem_bsr_pattern_device
+    fn = lib.fem_bsr_pattern_device_noalloc if noalloc else lib.fem_bsr_pattern_device
     _check(fn(num_nodes, num_elements, npe, _dptr(conn_ptr), _dptr(conn), _dptr(rowp), None, 0, ctypes.byref(nb),
               _stream(stream)))
     cols = torch.empty(max(nb.value, 1), dtype=torch.int32, device=conn.device)

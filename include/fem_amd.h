@@ -152,6 +152,16 @@ int fem_write_bdf(const char *path, int order, int numNodes, int numElements, co
  * elements hold more than 1024 node references (the host builder has no such limit). */
 int fem_bsr_pattern_device(int numNodes, int numElements, int nodesPerElement, const int *connPtr, const int *conn,
                            int *rowp, int *cols, int64_t colsCapacity, int64_t *numBlocks, void *stream);
+/* fem_bsr_pattern_device without any device allocation: its scratch (per-node counts and offsets, the
+ * node->element adjacency, row lengths, scan sums, error flags) lives in a fixed 64 MiB array the library holds
+ * on each GPU it runs on until the process ends. No counterpart in the reference: it is for callers that
+ * cannot allocate (tight serving loops, harnesses that forbid hipMalloc). The scratch is about 12 bytes per
+ * node plus 4 per element node reference (a 700 x 700 order-2 mesh fits, a 1000 x 1000 one does not); a mesh
+ * needing more than 64 MiB gets FEM_ERR_INVALID_ARGUMENT before any device work. Every call on a GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (other threads, other streams) race. Same results as fem_bsr_pattern_device. */
+int fem_bsr_pattern_device_noalloc(int numNodes, int numElements, int nodesPerElement, const int *connPtr,
+                                   const int *conn, int *rowp, int *cols, int64_t colsCapacity, int64_t *numBlocks,
+                                   void *stream);
 int fem_element_bcsr_map_device(const int *connPtr, const int *conn, int numElements, int nodesPerElement,
                                 int numRows, const int *rowp, const int *cols, int bsize, int *map, void *stream);
 /* Greedy element colouring: no two elements of one colour share a node. colour[e] in
