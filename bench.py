@@ -204,12 +204,15 @@ def bench_batch(args, F, torch):
 
     t_batch = time_steps(lambda: batch.assemble(precision=prec, stream=stream), args.steps)
     log(f"batch (autotuned plans): {t_batch * 1e3:.3f} ms per assembly of all {count} meshes")
-    scales = {}
+    scales, waves = {}, {}
     for it in items:
-        sc = it["plan"].info()["part_scale"]
+        inf = it["plan"].info()
         key = str(it["mesh"].order)
-        scales.setdefault(key, {}).setdefault(str(sc), 0)
-        scales[key][str(sc)] += 1
+        scales.setdefault(key, {}).setdefault(str(inf["part_scale"]), 0)
+        scales[key][str(inf["part_scale"])] += 1
+        waves.setdefault(key, {}).setdefault(str(inf["waves_per_group"]), 0)
+        waves[key][str(inf["waves_per_group"])] += 1
+    log(f"element-once waves per workgroup chosen per order (plan assembly): {waves}")
     log(f"part scales chosen per order: {scales}")
     ubatch = F.Batch(max_jobs=count, graph=graph)
     for it in items:
@@ -271,7 +274,8 @@ def bench_batch(args, F, torch):
                      "frac": b_alg / t_batch / 1e9 / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": b_alg},
         "baselines": {"one_plan_launch_per_mesh_ms": t_plans * 1e3, "plain_fp64_atomic_per_mesh_ms": t_plain * 1e3,
                       "speedup_vs_plain": t_plain / t_batch, "untuned_plans_batch_ms": t_untuned * 1e3},
-        "autotune": {"part_scales_by_order": scales, "candidates": [1.0, 0.5, 0.25]},
+        "autotune": {"part_scales_by_order": scales, "part_scale_candidates": [1.0, 0.5, 0.25],
+                     "waves_per_group_by_order": waves, "waves_candidates_orders_3_4": [1, 2, 4]},
         "cpu_baseline": None,
         "checks": check,
     }
@@ -582,7 +586,7 @@ def run_single(args, F, torch, workload, path, precision, strain="linear", autot
 
         def kernel():
             F.jacobian(mesh.order, conn_ptr, conn, mesh.num_elements, states, coords, E_MOD, NU, THICK, dmap, res,
-                       vals, precision=prec, stream=stream, nonlinear=strain == "nonlinear")
+                       vals, precision=prec, stream=stream, nonlinear=strain == "nonlinear", dropin=path == "dropin")
 
         def zero():
             res.zero_()
@@ -655,7 +659,9 @@ def run_single(args, F, torch, workload, path, precision, strain="linear", autot
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": plan_kernel_name(path, mesh.order) if plan is not None
-                     else ("element_kernel_nl" if strain == "nonlinear" else "element_kernel"),
+                     else ("element_kernel_nl" if strain == "nonlinear" else
+                           "owner_stream_kernel (accumulate)" if path == "dropin" and mesh.order <= 2
+                           else "element_kernel"),
                      "algorithmic_bytes": b_alg, "kernel_ms": kernel_s * 1e3},
         "reference_protocol": protocol, "checks": checks, "plan": info, "cpu_baseline": cpu_rec,
         "dtype": "f64" if prec == F.FEM_PREC_FP64 else "f32+f64acc",
@@ -684,14 +690,31 @@ def serve_leg(args, F, torch):
     return {k: rec[k] for k in keep}
 
 
+def batch_leg(args, F, torch):
+    """C4 leg of the default run: bench_batch (256 meshes, mixed precision, autotuned plans, graph replay)."""
+    import contextlib
+    import io
+
+    sub = argparse.Namespace(**vars(args))
+    sub.pool, sub.precision, sub.workload = 0, "mixed", "batch256-mixed"
+    buf = io.StringIO()
+    with contextlib.redirect_stdout(buf):
+        bench_batch(sub, F, torch)
+    rec = json.loads(buf.getvalue().strip().splitlines()[-1])
+    keep = ("value", "unit", "ms_per_step", "config", "roofline", "baselines", "autotune", "checks")
+    return {k: rec[k] for k in keep}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="square1000-p2", choices=sorted(WORKLOADS) + sorted(POOL_WORKLOADS))
-    ap.add_argument("--path", default="plan", choices=["plan", "plan-det", "plan-owner", "atomic"],
-                    help="plan-owner: orders 3-4 through the owner-computes parts (FEM_PLAN_OWNER_ONLY)")
+    ap.add_argument("--path", default="plan", choices=["plan", "plan-det", "plan-owner", "atomic", "dropin"],
+                    help="plan-owner: orders 3-4 through the owner-computes parts (FEM_PLAN_OWNER_ONLY); atomic: the "
+                         "element-parallel atomic kernels (fem_jacobian); dropin: the reference's entry point "
+                         "(runJacobianKernel -> fem_jacobian_dropin, a cached plan in accumulate mode at orders 1-2)")
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
                     help="default: fp64 for single meshes, mixed for batch256-mixed")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -709,8 +732,21 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="plan path: build the plan with FEM_PLAN_AUTOTUNE")
     ap.add_argument("--strain", default="linear", choices=["linear", "nonlinear"],
                     help="nonlinear: the Green-strain tangent (drop-in atomic path, FP64)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device twice (the known-size kernel the FETCH_SIZE / WRITE_SIZE "
+                         "counters are scaled by) and exit")
     args = ap.parse_args()
     args.serve_loads = (0.8, 0.5, 0.95)
+    if args.copy_calibration:
+        import torch
+
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(2):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 8, "copies": 2}), flush=True)
+        return
     if args.autotune and args.path != "plan":
         ap.error("--autotune applies to --path plan only")
     if args.strain == "nonlinear" and (args.path != "atomic" or args.precision == "mixed"):
@@ -742,9 +778,17 @@ def main():
         # order-4 annulus in both precisions, and a compact serving run against plain per-request launches.
         legs = {}
         for wl, path, prec in (("square1000-p2", "plan", "mixed"), ("square1000-p2", "atomic", "fp64"),
-                               ("annulus200x1250-p4", "plan", "fp64"), ("annulus200x1250-p4", "plan", "mixed")):
-            r = run_single(args, F, torch, wl, path, prec, ref_protocol=True, check=not args.no_check)
+                               ("square1000-p2", "dropin", "fp64"),
+                               ("annulus200x1250-p4", "plan", "fp64"), ("annulus200x1250-p4", "plan", "mixed"),
+                               ("annulus200x1250-p4", "atomic", "fp64"),
+                               ("annulus200x1250-p4", "plan-owner", "mixed"), ("square64-p1", "plan", "fp64"),
+                               ("square64-p1", "atomic", "fp64")):
+            r = run_single(args, F, torch, wl, path, prec, ref_protocol=True, check=not args.no_check,
+                           cpu=wl == "square64-p1" and not args.no_cpu_baseline)
             legs[f"{wl}/{path}/{prec}"] = r
+        log("batch leg")
+        legs["batch256-mixed"] = batch_leg(args, F, torch)
+        torch.cuda.empty_cache()  # the serving leg's server takes its budget from free device memory
         log("serving leg")
         legs["serving"] = serve_leg(args, F, torch)
     r = main_rec

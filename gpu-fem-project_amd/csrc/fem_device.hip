@@ -348,7 +348,8 @@ __device__ __forceinline__ void stage_stiffness(WaveLDS<P, T> &W, const double2 
       if (valid) W.sg[buf][el][b] = v1;
       wave_sync();
 #pragma unroll
-      for (int d = 0; d < D; d++) wn[d] = W.sg[buf][el][arow[d]];
+      for (int d = 1; d < D; d++) wn[d] = W.sg[buf][el][arow[d]];
+      wn[0] = v1;  // the lane's own row (arow[0] == b): its value is in registers
     }
     accumulate(wc, nx, ny);
     nx = nx1;
@@ -367,7 +368,8 @@ __device__ __forceinline__ void stage_stiffness(WaveLDS<P, T> &W, const double2 
     if (valid) W.sg[0][el][b] = v;
     wave_sync();
 #pragma unroll
-    for (int d = 0; d < D; d++) wA[d] = W.sg[0][el][arow[d]];
+    for (int d = 1; d < D; d++) wA[d] = W.sg[0][el][arow[d]];
+    wA[0] = v;
   }
   // Points 0..Q-2 publish their successor; unrolled by two so that the register sets alternate instead
   // of being copied, with the odd step and the last point peeled at compile time.
@@ -608,6 +610,7 @@ struct SplitJob {
   double *stage, *rstage;
   Material mat;
   int count, numBlocks, numNodes, numGather, numRgather;
+  int accumulate;  // 1: outputs are added to (+=), staging is still written
 };
 
 // Elements first .. first+EPW-1 of `job` (those below job.count) on one wavefront: geometry, Gram tiles,
@@ -694,6 +697,10 @@ __device__ __forceinline__ void tiled_wave(WaveLDSTile<P, T> &W, const Basis1D &
   if (RES && nvalid) {
     const int rc = job.rcode[(first + el) * N + a];
     double *r = rc < job.numNodes ? job.residual + 2 * int64_t(rc) : job.rstage + 2 * (int64_t(rc) - job.numNodes);
+    if (job.accumulate && rc < job.numNodes) {
+      Rx += r[0];
+      Ry += r[1];
+    }
     __builtin_nontemporal_store(Rx, r);
     __builtin_nontemporal_store(Ry, r + 1);
   }
@@ -724,10 +731,13 @@ __device__ __forceinline__ void tiled_wave(WaveLDSTile<P, T> &W, const Basis1D &
   constexpr int CHK = EP::CHK;
   const int c = lane & 3;
   const int lastEl = int(min<int64_t>(EPW, job.count - first)) - 1;
-  // block (e, rem) of store instruction `it` of group R (rem: offset within the element's rows of the group)
-  auto locate = [&](int R, int it, int &e, int &rem) __attribute__((always_inline)) {
+  // block (e, rem) of store instruction `it` of group R (rem: offset within the element's rows of the group);
+  // dup: the lane repeats another lane's block (past the group's last block or the wavefront's last element)
+  auto locate = [&](int R, int it, int &e, int &rem, bool &dup) __attribute__((always_inline)) {
     const int groupBlocks = (N - 4 * R < 4 ? N - 4 * R : 4) * N;
-    rem = min(16 * it + (lane >> 2), EPW * groupBlocks - 1);
+    const int want = 16 * it + (lane >> 2);
+    dup = want > EPW * groupBlocks - 1;
+    rem = min(want, EPW * groupBlocks - 1);
     e = 0;
 #pragma unroll
     for (int k = 1; k < EPW; k++) {
@@ -739,6 +749,7 @@ __device__ __forceinline__ void tiled_wave(WaveLDSTile<P, T> &W, const Basis1D &
     if (e > lastEl) {
       e = lastEl;
       rem = groupBlocks - 1;
+      dup = true;
     }
   };
   int cur[CHK], nxt[CHK];
@@ -749,7 +760,8 @@ __device__ __forceinline__ void tiled_wave(WaveLDSTile<P, T> &W, const Basis1D &
     for (int it = 0; it < CHK; it++) {
       if (ch * CHK + it >= EP::iters(R)) break;
       int e, rem;
-      locate(R, ch * CHK + it, e, rem);
+      bool dup;
+      locate(R, ch * CHK + it, e, rem, dup);
       at[it] = job.code[((first + e) * N + 4 * R) * N + rem];
     }
   };
@@ -778,11 +790,19 @@ __device__ __forceinline__ void tiled_wave(WaveLDSTile<P, T> &W, const Basis1D &
     for (int it = 0; it < CHK; it++) {
       if (ch * CHK + it >= EP::iters(R)) break;
       int e, rem;
-      locate(R, ch * CHK + it, e, rem);
+      bool dup;
+      locate(R, ch * CHK + it, e, rem, dup);
       const int at = cur[it];
       const int i = rem / N, b = rem - i * N;
       double *blkp = at < job.numBlocks ? job.values + 4 * int64_t(at) : job.stage + 4 * (int64_t(at) - job.numBlocks);
-      __builtin_nontemporal_store(double(W.get(e, i, b, c)), blkp + c);
+      double val = double(W.get(e, i, b, c));
+      if (!job.accumulate) {
+        __builtin_nontemporal_store(val, blkp + c);
+      } else if (!dup) {
+        // accumulate mode: each block added once (repeating lanes store nothing); staging is written
+        if (at < job.numBlocks) val += blkp[c];
+        __builtin_nontemporal_store(val, blkp + c);
+      }
     }
 #pragma unroll
     for (int it = 0; it < CHK; it++) cur[it] = nxt[it];
@@ -1073,6 +1093,8 @@ struct MeshJob {
   double *values;         // may be NULL
   double mu, lambda, t;
   int numParts, numColours;
+  int accumulate;  // 1: the write-out adds to the outputs (+=) instead of storing them
+  int pad_;
 };
 
 struct OwnerArgs {
@@ -1532,7 +1554,13 @@ __device__ __forceinline__ void owner_write(const MeshJob &J, double *acc, doubl
         // 1 KB contiguously.
         for (int i = lane; i < 2 * run.len; i += 64) {
           double *src = acc + 2 * (i & 1) * acc_plane<MAXB>() + run.lds + (i >> 1);
-          store_once(dst + i, src[0], src[acc_plane<MAXB>()]);
+          double v0 = src[0], v1 = src[acc_plane<MAXB>()];
+          if (J.accumulate) {
+            const double2 old = dst[i];
+            v0 += old.x;
+            v1 += old.y;
+          }
+          store_once(dst + i, v0, v1);
           if (ZERO) {
             src[0] = 0;
             src[acc_plane<MAXB>()] = 0;
@@ -1542,8 +1570,18 @@ __device__ __forceinline__ void owner_write(const MeshJob &J, double *acc, doubl
         // One block per lane (two 16-byte stores).
         double *src = acc + run.lds;
         for (int i = lane; i < run.len; i += 64) {
-          store_once(dst + 2 * i, src[i], src[i + acc_plane<MAXB>()]);
-          store_once(dst + 2 * i + 1, src[i + 2 * acc_plane<MAXB>()], src[i + 3 * acc_plane<MAXB>()]);
+          double v[4];
+#pragma unroll
+          for (int c = 0; c < 4; c++) v[c] = src[i + c * acc_plane<MAXB>()];
+          if (J.accumulate) {
+            const double2 o0 = dst[2 * i], o1 = dst[2 * i + 1];
+            v[0] += o0.x;
+            v[1] += o0.y;
+            v[2] += o1.x;
+            v[3] += o1.y;
+          }
+          store_once(dst + 2 * i, v[0], v[1]);
+          store_once(dst + 2 * i + 1, v[2], v[3]);
           if (ZERO) {
 #pragma unroll
             for (int c = 0; c < 4; c++) src[i + c * acc_plane<MAXB>()] = 0;
@@ -1555,7 +1593,13 @@ __device__ __forceinline__ void owner_write(const MeshJob &J, double *acc, doubl
   if (RES && J.residual) {
     const gptr_mut<double2> res = gmem_mut(reinterpret_cast<double2 *>(J.residual));
     for (int i = int(threadIdx.x); i < nr; i += WAVES * 64) {
-      res[lrows[i]] = reinterpret_cast<const double2 *>(racc)[i];
+      double2 r = reinterpret_cast<const double2 *>(racc)[i];
+      if (J.accumulate) {
+        const double2 old = res[lrows[i]];
+        r.x += old.x;
+        r.y += old.y;
+      }
+      res[lrows[i]] = r;
       if (ZERO) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
     }
   }
@@ -1921,10 +1965,17 @@ void launch_element_order(int order, const ElemArgs &a, int precision, hipStream
 constexpr int kTiledWaves = 4;
 
 template <int P, typename T, bool RES>
-void launch_element_split(const SplitJob &job, const Basis1D &basis, hipStream_t stream) {
-  constexpr int per = kTiledWaves * Ord<P>::EPW;
-  const int grid = (job.count + per - 1) / per;
-  if (grid > 0) element_tiled_kernel<P, T, kTiledWaves, RES><<<grid, kTiledWaves * 64, 0, stream>>>(job, basis);
+void launch_element_split(const SplitJob &job, const Basis1D &basis, int waves, hipStream_t stream) {
+  auto go = [&](auto W) {
+    constexpr int WAVES = decltype(W)::value;
+    constexpr int per = WAVES * Ord<P>::EPW;
+    const int grid = (job.count + per - 1) / per;
+    if (grid > 0) element_tiled_kernel<P, T, WAVES, RES><<<grid, WAVES * 64, 0, stream>>>(job, basis);
+  };
+  // wavefronts per workgroup: the plan's (autotuned) choice
+  if (waves == 1) go(std::integral_constant<int, 1>{});
+  else if (waves == 2) go(std::integral_constant<int, 2>{});
+  else go(std::integral_constant<int, kTiledWaves>{});
 }
 
 // Second pass of the element-once assembly: output dst[i] (C doubles from C * dst[i]) is the sum of the
@@ -1932,27 +1983,32 @@ void launch_element_split(const SplitJob &job, const Basis1D &basis, hipStream_t
 // component t % C of output t / C, so a wavefront reads consecutive slots.
 template <int C>
 __device__ __forceinline__ void gather_one(const int *dst, const int *ptr, const double *stage, double *out, int64_t i,
-                                           int c) {
+                                           int c, bool accumulate = false) {
   const int k0 = ptr[i], k1 = ptr[i + 1];
   double sum = 0;
   for (int k = k0; k < k1; k++) sum += __builtin_nontemporal_load(stage + int64_t(C) * k + c);
-  __builtin_nontemporal_store(sum, out + int64_t(C) * dst[i] + c);
+  double *o = out + int64_t(C) * dst[i] + c;
+  if (accumulate) sum += *o;
+  __builtin_nontemporal_store(sum, o);
 }
 
 template <int C>
 __global__ __launch_bounds__(256) void gather_shared(const int *dst, const int *ptr, const double *stage,
-                                                     double *out, int n) {
+                                                     double *out, int n, int accumulate) {
   const int64_t total = int64_t(n) * C, stride = int64_t(gridDim.x) * 256;
   for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total; t += stride) {
-    gather_one<C>(dst, ptr, stage, out, t / C, int(t % C));
+    gather_one<C>(dst, ptr, stage, out, t / C, int(t % C), accumulate != 0);
   }
 }
 
 template <int C>
-void launch_gather_shared(const int *dst, const int *ptr, const double *stage, double *out, int n,
+void launch_gather_shared(const int *dst, const int *ptr, const double *stage, double *out, int n, int accumulate,
                           hipStream_t stream) {
   const int64_t blocks = (int64_t(n) * C + 255) / 256;
-  if (n > 0) gather_shared<C><<<unsigned(std::min<int64_t>(blocks, 1 << 20)), 256, 0, stream>>>(dst, ptr, stage, out, n);
+  if (n > 0) {
+    gather_shared<C><<<unsigned(std::min<int64_t>(blocks, 1 << 20)), 256, 0, stream>>>(dst, ptr, stage, out, n,
+                                                                                        accumulate);
+  }
 }
 
 // The gathers of a batch's element-once jobs: C = 4, the staged blocks (outStart[j]: job j's first output
@@ -1974,17 +2030,17 @@ __global__ __launch_bounds__(256) void gather_batch(const SplitJob *jobs, const 
 // Element-once assembly of one plan: the element kernel stores exclusive outputs and stages the rest, then
 // the gathers store the staged sums.
 template <int P>
-void launch_split(const SplitJob &job, const Basis1D &basis, int precision, hipStream_t s) {
+void launch_split(const SplitJob &job, const Basis1D &basis, int precision, int waves, hipStream_t s) {
   const bool res = job.residual != nullptr;
   if (precision == FEM_PREC_MIXED) {
-    if (res) launch_element_split<P, float, true>(job, basis, s);
-    else launch_element_split<P, float, false>(job, basis, s);
+    if (res) launch_element_split<P, float, true>(job, basis, waves, s);
+    else launch_element_split<P, float, false>(job, basis, waves, s);
   } else {
-    if (res) launch_element_split<P, double, true>(job, basis, s);
-    else launch_element_split<P, double, false>(job, basis, s);
+    if (res) launch_element_split<P, double, true>(job, basis, waves, s);
+    else launch_element_split<P, double, false>(job, basis, waves, s);
   }
-  launch_gather_shared<4>(job.gdst, job.gptr, job.stage, job.values, job.numGather, s);
-  if (res) launch_gather_shared<2>(job.rdst, job.rptr, job.rstage, job.residual, job.numRgather, s);
+  launch_gather_shared<4>(job.gdst, job.gptr, job.stage, job.values, job.numGather, job.accumulate, s);
+  if (res) launch_gather_shared<2>(job.rdst, job.rptr, job.rstage, job.residual, job.numRgather, job.accumulate, s);
 }
 
 template <int P, bool RES, bool JAC>
@@ -2459,6 +2515,31 @@ int fem_plan_create(int order, int numNodes, int numElements, const int *connPtr
       fem_plan_destroy(best);
       return rc;
     }
+    if (order >= 3 && cand->dConn) {
+      // orders 3-4: the element-once kernel's wavefronts per workgroup (lanes per element and elements per
+      // wavefront are fixed by the order: N and 64 / N)
+      int bestWaves = cand->tiledWaves;
+      for (const int waves : {1, 2, kTiledWaves}) {
+        cand->tiledWaves = waves;
+        double s2 = 0, s2Min = 1e30;
+        for (int rep = 0; rep < 3 && !rc; rep++) {
+          rc = fem_plan_assemble(cand, tb.states, tb.coords, 1.0, 0.3, 1.0, tb.res, tb.vals, FEM_PREC_MIXED,
+                                 tb.stream, &s2);
+          if (rep > 0) s2Min = std::min(s2Min, s2);
+        }
+        if (rc) break;
+        if (s2Min < secMin) {
+          secMin = s2Min;
+          bestWaves = waves;
+        }
+      }
+      cand->tiledWaves = bestWaves;
+      if (rc) {
+        fem_plan_destroy(cand);
+        fem_plan_destroy(best);
+        return rc;
+      }
+    }
     cand->tunedSeconds = secMin;
     if (!best || secMin < bestSeconds) {
       fem_plan_destroy(best);
@@ -2479,6 +2560,16 @@ int fem_plan_tuning(const fem_plan *plan, double *partScale, double *seconds) {
   return FEM_OK;
 }
 
+int fem_plan_tuning_shape(const fem_plan *plan, int *wavesPerGroup, int *elementsPerWave, int *lanesPerElement) {
+  if (!plan) return set_error(FEM_ERR_NULL_POINTER, "plan is NULL");
+  const int N = nodes_per_element(plan->order);
+  const bool tiled = plan->dConn != nullptr;  // orders 3-4 element-once Jacobians
+  if (wavesPerGroup) *wavesPerGroup = tiled ? plan->tiledWaves : owner_waves(plan->order);
+  if (elementsPerWave) *elementsPerWave = 64 / N;
+  if (lanesPerElement) *lanesPerElement = N;
+  return FEM_OK;
+}
+
 int fem_plan_info(const fem_plan *plan, int64_t *numParts, int64_t *numBlocks, int64_t *elementVisits,
                   int64_t *deviceBytes) {
   if (!plan) return set_error(FEM_ERR_NULL_POINTER, "plan is NULL");
@@ -2491,6 +2582,22 @@ int fem_plan_info(const fem_plan *plan, int64_t *numParts, int64_t *numBlocks, i
 
 int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCoords, double E, double nu, double t,
                       double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds) {
+  return fem::plan_assemble(plan, states, nodeCoords, E, nu, t, residual, matEntries, precision, stream,
+                            elapsedSeconds, false);
+}
+
+int fem_plan_assemble_accumulate(fem_plan *plan, const double *states, const double *nodeCoords, double E, double nu,
+                                 double t, double *residual, double *matEntries, int precision, void *stream,
+                                 double *elapsedSeconds) {
+  return fem::plan_assemble(plan, states, nodeCoords, E, nu, t, residual, matEntries, precision, stream,
+                            elapsedSeconds, true);
+}
+
+}  // extern "C"
+
+int fem::plan_assemble(fem_plan *plan, const double *states, const double *nodeCoords, double E, double nu, double t,
+                       double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds,
+                       bool accumulate) {
   if (!plan) return set_error(FEM_ERR_NULL_POINTER, "plan is NULL");
   if (!states || !nodeCoords) return set_error(FEM_ERR_NULL_POINTER, "states/nodeCoords is NULL");
   if (!residual && !matEntries) return set_error(FEM_ERR_NULL_POINTER, "residual and matEntries are both NULL");
@@ -2508,7 +2615,8 @@ int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCo
   if ((rc = make_basis(plan->order, L.args.basis))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   // The device job record is rewritten only when the request's pointers or material change.
-  const MeshJob job = make_job(plan, states, nodeCoords, residual, matEntries, m);
+  MeshJob job = make_job(plan, states, nodeCoords, residual, matEntries, m);
+  job.accumulate = accumulate ? 1 : 0;
   if (!plan->jobUploaded || std::memcmp(&job, plan->hostJob, sizeof(MeshJob)) != 0) {
     FEM_HIP_CHECK(hipMemcpyAsync(plan->dJob, &job, sizeof(MeshJob), hipMemcpyHostToDevice, s));
     std::memcpy(plan->hostJob, &job, sizeof(MeshJob));
@@ -2528,10 +2636,11 @@ int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCo
   if (plan->dConn && matEntries) {
     // Element-once assembly (orders 3-4): exclusive outputs stored by their element, the rest staged and
     // gathered. Deterministic, so it also serves FEM_PLAN_DETERMINISTIC plans.
-    const SplitJob job = make_split_job(plan, states, nodeCoords, residual, matEntries, m);
+    SplitJob job = make_split_job(plan, states, nodeCoords, residual, matEntries, m);
+    job.accumulate = accumulate ? 1 : 0;
     return timed(s, elapsedSeconds, [&]() {
-      if (plan->order == 3) launch_split<3>(job, L.args.basis, precision, s);
-      else launch_split<4>(job, L.args.basis, precision, s);
+      if (plan->order == 3) launch_split<3>(job, L.args.basis, precision, plan->tiledWaves, s);
+      else launch_split<4>(job, L.args.basis, precision, plan->tiledWaves, s);
       return FEM_OK;
     });
   }
@@ -2540,6 +2649,8 @@ int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCo
     return FEM_OK;
   });
 }
+
+extern "C" {
 
 int fem_plan_destroy(fem_plan *plan) {
   if (!plan) return FEM_OK;

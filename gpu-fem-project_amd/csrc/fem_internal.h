@@ -104,6 +104,10 @@ bool plan_streams_public(const fem_plan *plan);
 // Records a graph batch's launch graph for `precision` now (it does not depend on the batch's contents), so that
 // no capture is in flight later, when other threads may be using the device. No-op without FEM_BATCH_GRAPH.
 int batch_prepare_graph(fem_batch *batch, int precision);
+// fem_plan_assemble (accumulate false) and fem_plan_assemble_accumulate (true).
+int plan_assemble(fem_plan *plan, const double *states, const double *nodeCoords, double E, double nu, double t,
+                  double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds,
+                  bool accumulate);
 
 }  // namespace fem
 
@@ -141,6 +145,7 @@ struct fem_plan {
   int64_t numSlots = 0, numRslots = 0;  // staged contributions: blocks, residual entries
   double partScale = 1.0;    // part budgets relative to the kernel's (FEM_PLAN_AUTOTUNE picks it)
   double tunedSeconds = 0;   // measured assembly time of the chosen build (autotuned plans)
+  int tiledWaves = 4;        // orders 3-4: wavefronts per workgroup of the element-once kernel (autotuned)
   void *dJob = nullptr;      // device job record of the last assembly
   alignas(8) unsigned char hostJob[256] = {};  // host copy of that record
   bool jobUploaded = false;

@@ -180,9 +180,23 @@ void pool_free(fem_server *s, void *p, size_t bytes) {
   s->freeRanges.emplace(off, len);
 }
 
+void finish_request(fem_server *s, Request *r, double t, int status, const char *err);
+
 // Admits queued requests in FIFO order while their outputs fit the budget. Caller holds mu.
 void admit_into(fem_server *s, std::vector<Request *> &out, int capacity) {
   int perGroup[12] = {0};
+  if (!s->queue.empty() && !ensure_arena(s)) {
+    // The arena could not be created (device memory taken by others): nothing can ever be admitted, so the
+    // queued requests fail instead of waiting.
+    const double t = now_seconds();
+    while (!s->queue.empty()) {
+      finish_request(s, s->queue.front(), t, FEM_ERR_OUT_OF_MEMORY,
+                     "the server's output arena (the budget left after resident meshes) could not be allocated");
+      s->queue.pop_front();
+    }
+    s->doneCv.notify_all();
+    return;
+  }
   while (!s->queue.empty() && int(out.size()) < capacity) {
     Request *r = s->queue.front();
     const ServerMesh &m = r->m;

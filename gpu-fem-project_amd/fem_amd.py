@@ -74,6 +74,9 @@ SIGNATURES = {
     "fem_element_colouring": ([_I, _I, _I, _P, _P, _P, ctypes.POINTER(_I)], _I),
     "fem_residual": ([_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _I, _P, ctypes.POINTER(_D)], _I),
     "fem_jacobian": ([_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _P, _P, _I, _P, ctypes.POINTER(_D)], _I),
+    "fem_jacobian_dropin": ([_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _P, _P, _I, _P, ctypes.POINTER(_D)], _I),
+    "fem_dropin_cache_clear": ([], _I),
+    "fem_dropin_cache_info": ([_I, _P, _P, _I, _P, ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "fem_residual_nonlinear": ([_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _I, _P, ctypes.POINTER(_D)], _I),
     "fem_jacobian_nonlinear": ([_I, _P, _P, _I, _P, _P, _D, _D, _D, _P, _P, _P, _I, _P, ctypes.POINTER(_D)], _I),
     "fem_jacobian_coloured": (
@@ -83,7 +86,9 @@ SIGNATURES = {
     "fem_plan_create": ([_I, _I, _I, _P, _P, _P, _P, _P, _I, ctypes.POINTER(_P)], _I),
     "fem_plan_info": ([_P, _PI64, _PI64, _PI64, _PI64], _I),
     "fem_plan_tuning": ([_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], _I),
+    "fem_plan_tuning_shape": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "fem_plan_assemble": ([_P, _P, _P, _D, _D, _D, _P, _P, _I, _P, ctypes.POINTER(_D)], _I),
+    "fem_plan_assemble_accumulate": ([_P, _P, _P, _D, _D, _D, _P, _P, _I, _P, ctypes.POINTER(_D)], _I),
     "fem_plan_destroy": ([_P], _I),
     "fem_bsr_spmv": ([_I, _P, _P, _P, _P, _P, _P], _I),
     "fem_batch_create": ([_I, _I, ctypes.POINTER(_P)], _I),
@@ -403,14 +408,30 @@ def residual(order, conn_ptr, conn, num_elements, states, coords, E, nu, t, out,
 
 
 def jacobian(order, conn_ptr, conn, num_elements, states, coords, E, nu, t, bcsr_map, res, values,
-             precision=FEM_PREC_FP64, stream=None, timed=False, nonlinear=False):
-    """values += K through the map; res (optional) += R. nonlinear: the NONLINEAR (Green) strain's tangent."""
+             precision=FEM_PREC_FP64, stream=None, timed=False, nonlinear=False, dropin=False):
+    """values += K through the map; res (optional) += R. nonlinear: the NONLINEAR (Green) strain's tangent.
+    dropin: through fem_jacobian_dropin (cached resident plan in accumulate mode where the map allows)."""
     sec = ctypes.c_double()
-    fn = load().fem_jacobian_nonlinear if nonlinear else load().fem_jacobian
+    if nonlinear and dropin:
+        raise ValueError("the drop-in plan cache is LINEAR only")
+    fn = (load().fem_jacobian_nonlinear if nonlinear else
+          load().fem_jacobian_dropin if dropin else load().fem_jacobian)
     _check(fn(order, _dptr(conn_ptr), _dptr(conn), num_elements, _dptr(states), _dptr(coords), E, nu, t,
               _dptr(bcsr_map), _dptr(res), _dptr(values), precision, _stream(stream),
               ctypes.byref(sec) if timed else None))
     return sec.value if timed else None
+
+
+def dropin_cache_clear():
+    _check(load().fem_dropin_cache_clear())
+
+
+def dropin_cache_info(order, conn_ptr, conn, num_elements, bcsr_map):
+    """(cached, served by a plan) for a drop-in combination."""
+    c, u = ctypes.c_int(), ctypes.c_int()
+    _check(load().fem_dropin_cache_info(order, _dptr(conn_ptr), _dptr(conn), num_elements, _dptr(bcsr_map),
+                                        ctypes.byref(c), ctypes.byref(u)))
+    return bool(c.value), bool(u.value)
 
 
 def jacobian_coloured(order, conn_ptr, conn, num_elements, states, coords, E, nu, t, bcsr_map, colour_elems,
@@ -451,14 +472,18 @@ class Plan:
         scale, sec = ctypes.c_double(), ctypes.c_double()
         _check(self._lib.fem_plan_tuning(self.handle, ctypes.byref(scale), ctypes.byref(sec)))
         out["part_scale"] = scale.value
+        shape = [ctypes.c_int() for _ in range(3)]
+        _check(self._lib.fem_plan_tuning_shape(self.handle, *[ctypes.byref(v) for v in shape]))
+        out["waves_per_group"], out["elements_per_wave"], out["lanes_per_element"] = [v.value for v in shape]
         return out
 
     def assemble(self, states, coords, E, nu, t, res=None, values=None, precision=FEM_PREC_FP64, stream=None,
-                 timed=False):
+                 timed=False, accumulate=False):
+        """Writes (accumulate: adds to) the residual and / or BSR values."""
         sec = ctypes.c_double()
-        _check(self._lib.fem_plan_assemble(self.handle, _dptr(states), _dptr(coords), E, nu, t, _dptr(res),
-                                           _dptr(values), precision, _stream(stream),
-                                           ctypes.byref(sec) if timed else None))
+        fn = self._lib.fem_plan_assemble_accumulate if accumulate else self._lib.fem_plan_assemble
+        _check(fn(self.handle, _dptr(states), _dptr(coords), E, nu, t, _dptr(res), _dptr(values), precision,
+                  _stream(stream), ctypes.byref(sec) if timed else None))
         return sec.value if timed else None
 
     def close(self):

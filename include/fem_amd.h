@@ -182,6 +182,23 @@ int fem_residual(int order, const int *connPtr, const int *conn, int numElements
 int fem_jacobian(int order, const int *connPtr, const int *conn, int numElements, const double *states,
                  const double *nodeCoords, double E, double nu, double t, const int *elementBCSRMap,
                  double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds);
+/* fem_jacobian with the same arguments and semantics (+= into caller-owned residual / matEntries), for callers
+ * of the reference's interface (runJacobianKernel, element/FEKernels.h:1125-1225): on first use of a (connPtr,
+ * conn, elementBCSRMap, numElements, order) combination the library builds the connectivity's BSR pattern and
+ * element map on the device and, if that map equals the caller's, a resident plan (kept until
+ * fem_dropin_cache_clear; at most four combinations, least recently used evicted). Calls then run the plan in
+ * accumulate mode (fem_plan_assemble_accumulate): each output is read and written once, without atomics. Each
+ * call first checks a 64-bit checksum of conn and of every 61st map entry (a change rebuilds the entry) and
+ * synchronises the stream. Another matrix layout, nodes no element references, or a failed build run
+ * fem_jacobian. elapsedSeconds covers the assembly. Not for graph capture (it synchronises). */
+int fem_jacobian_dropin(int order, const int *connPtr, const int *conn, int numElements, const double *states,
+                        const double *nodeCoords, double E, double nu, double t, const int *elementBCSRMap,
+                        double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds);
+int fem_dropin_cache_clear(void);
+/* Whether the combination is cached (*cached) and served by a plan (*usesPlan; else by fem_jacobian). */
+int fem_dropin_cache_info(int order, const int *connPtr, const int *conn, int numElements, const int *elementBCSRMap,
+                          int *cached, int *usesPlan);
+
 /* NONLINEAR (Green) strain versions of fem_residual / fem_jacobian (same arguments and accumulate semantics):
  * the reference's kernels instantiated with GreenStrainType::NONLINEAR (element/FEKernels.h:286-315,
  * 585-1006; its dispatchers runResidualKernel / runJacobianKernel, :1030-1136, fix LINEAR). The stiffness is
@@ -231,6 +248,11 @@ int fem_plan_create(int order, int numNodes, int numElements, const int *connPtr
  * plan, the measured seconds of the chosen build (0 otherwise). No counterpart in the reference: the
  * reference fixes elements per block at compile time (FEKernels.h:1015-1116). */
 int fem_plan_tuning(const fem_plan *plan, double *partScale, double *seconds);
+/* Launch shape of the plan's Jacobian kernel: wavefronts per workgroup (autotuned at orders 3-4 among 1, 2 and
+ * 4 for the element-once kernel; the owner kernel's fixed count at orders 1-2), elements per wavefront and
+ * lanes per element (64 / N and N, fixed by the order). The reference tunes elements per block by hand per
+ * order (element/FEKernels.h:1009-1028, 1104-1123). */
+int fem_plan_tuning_shape(const fem_plan *plan, int *wavesPerGroup, int *elementsPerWave, int *lanesPerElement);
 
 /* What a plan holds: number of parts, BSR blocks, element visits (elements counted once per part
  * they touch), resident device bytes. */
@@ -239,6 +261,11 @@ int fem_plan_info(const fem_plan *plan, int64_t *numParts, int64_t *numBlocks, i
 /* residual (may be NULL) and matEntries (may be NULL) are overwritten. At least one must be given. */
 int fem_plan_assemble(fem_plan *plan, const double *states, const double *nodeCoords, double E, double nu, double t,
                       double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds);
+/* fem_plan_assemble that adds its sums to residual / matEntries (+=, the reference kernels' semantics) instead of
+ * storing them: each output is still read and written once (element-once plans still write their own staging). */
+int fem_plan_assemble_accumulate(fem_plan *plan, const double *states, const double *nodeCoords, double E, double nu,
+                                 double t, double *residual, double *matEntries, int precision, void *stream,
+                                 double *elapsedSeconds);
 int fem_plan_destroy(fem_plan *plan);
 
 /* ---- batches of resident plans ----------------------------------------------------------------

@@ -205,16 +205,34 @@ def test_elapsed_time_reported(fem, torch_gpu, cases):
     assert 0 < sec < 1.0
 
 
-def test_cpp_dropin_demo_runs():
+def test_cpp_dropin_demo_runs(fem, orc, tmp_path):
+    """The C++ program calling runResidualKernel / runJacobianKernel as the reference's harness does: its written
+    outputs (the reference's .csv / .mtx formats) against the oracle on the same generated mesh."""
     import os
     import subprocess
 
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpu-fem-project_amd",
                        "fem_dropin_demo")
     for order in (1, 2, 3, 4):
-        out = subprocess.run([exe, str(order), "12"], capture_output=True, text=True, timeout=120)
+        prefix = os.path.join(str(tmp_path), f"p{order}")
+        out = subprocess.run([exe, str(order), "12", prefix], capture_output=True, text=True, timeout=120)
         print(out.stdout)
         assert out.returncode == 0, out.stdout + out.stderr
+        m = fem.square_mesh(12, order, jitter=0.2, seed=2)
+        u = fem.fill_uniform(2 * m.num_nodes, -1e-3, 1e-3, 3)
+        pat = fem.bsr_pattern(m)
+        rref, vref = orc.jacobian(m, u, E, NU, T, fem.element_bcsr_map(m, pat), pat.num_blocks)
+        res = np.loadtxt(prefix + "_residual.csv")
+        mtx = np.loadtxt(prefix + "_jacobian.mtx", skiprows=1)
+        i, j = mtx[:, 0].astype(np.int64), mtx[:, 1].astype(np.int64)
+        br, bc = i // 2, j // 2
+        slot = np.array([pat.rowp[r] + np.searchsorted(pat.cols[pat.rowp[r]:pat.rowp[r + 1]], c)
+                         for r, c in zip(br, bc)])
+        got = np.zeros_like(vref)
+        got[4 * slot + 2 * (i % 2) + (j % 2)] = mtx[:, 2]
+        assert len(mtx) == vref.size
+        assert textbook.normwise_rel(res, rref) < FP64_TOL
+        assert textbook.normwise_rel(got, vref) < FP64_TOL
 
 
 @pytest.mark.parametrize("order", [1, 2, 3, 4])
