@@ -615,7 +615,7 @@ struct SplitJob {
 
 // Elements first .. first+EPW-1 of `job` (those below job.count) on one wavefront: geometry, Gram tiles,
 // residual and emission. Every lane of the wavefront calls it; it synchronises the wavefront only.
-template <int P, typename T, bool RES>
+template <int P, typename T, bool RES, bool ACC = false>
 __device__ __forceinline__ void tiled_wave(WaveLDSTile<P, T> &W, const Basis1D &basis, const double2 *gref,
                                            const SplitJob &job, int64_t first, int lane) {
   using O = Ord<P>;
@@ -697,7 +697,7 @@ __device__ __forceinline__ void tiled_wave(WaveLDSTile<P, T> &W, const Basis1D &
   if (RES && nvalid) {
     const int rc = job.rcode[(first + el) * N + a];
     double *r = rc < job.numNodes ? job.residual + 2 * int64_t(rc) : job.rstage + 2 * (int64_t(rc) - job.numNodes);
-    if (job.accumulate && rc < job.numNodes) {
+    if (ACC && rc < job.numNodes) {
       Rx += r[0];
       Ry += r[1];
     }
@@ -796,7 +796,7 @@ __device__ __forceinline__ void tiled_wave(WaveLDSTile<P, T> &W, const Basis1D &
       const int i = rem / N, b = rem - i * N;
       double *blkp = at < job.numBlocks ? job.values + 4 * int64_t(at) : job.stage + 4 * (int64_t(at) - job.numBlocks);
       double val = double(W.get(e, i, b, c));
-      if (!job.accumulate) {
+      if (!ACC) {
         __builtin_nontemporal_store(val, blkp + c);
       } else if (!dup) {
         // accumulate mode: each block added once (repeating lanes store nothing); staging is written
@@ -810,7 +810,7 @@ __device__ __forceinline__ void tiled_wave(WaveLDSTile<P, T> &W, const Basis1D &
 }
 
 
-template <int P, typename T, int WAVES, bool RES>
+template <int P, typename T, int WAVES, bool RES, bool ACC = false>
 __global__ __launch_bounds__(WAVES * 64) void element_tiled_kernel(const SplitJob job, const Basis1D basisIn) {
   using O = Ord<P>;
   __shared__ WaveLDSTile<P, T> wl[WAVES];
@@ -820,7 +820,7 @@ __global__ __launch_bounds__(WAVES * 64) void element_tiled_kernel(const SplitJo
   fill_gref<P>(gref, basisIn);
   __syncthreads();
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  tiled_wave<P, T, RES>(wl[wave], basis, gref, job, (int64_t(xcd_block()) * WAVES + wave) * O::EPW, lane);
+  tiled_wave<P, T, RES, ACC>(wl[wave], basis, gref, job, (int64_t(xcd_block()) * WAVES + wave) * O::EPW, lane);
 }
 
 // Index j of the last start[j] <= x over start[0..n) (start ascending, start[0] = 0 <= x).
@@ -1540,9 +1540,64 @@ __device__ __forceinline__ Run unpack_run(const int2 &r) { return Run{r.x & 0xff
 // lane moves a whole block (faster at orders 3-4, whose write-out loops are shorter). With ZERO, every
 // value read is cleared, so the accumulators are all zero again afterwards (blocks past the part's count
 // were never touched).
+// Accumulate-mode write-out (J.accumulate: the reference kernels' += semantics). Lane pairs move block halves
+// as in the HALVES form; each lane takes up to eight store slots per round and loads all their old values
+// before adding and storing any, so a round's loads are in flight together.
+template <bool RES, bool JAC, int WAVES, int MAXB, bool ZERO, typename RunT>
+__device__ void owner_write_accumulate(const MeshJob &J, double *acc, double *racc, const RunT *lruns, int nruns,
+                                       const int *lrows, int nr) {
+  constexpr int U = 8;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (JAC && J.values) {
+    const gptr_mut<double2> values = gmem_mut(reinterpret_cast<double2 *>(J.values));
+    for (int r = wave; r < nruns; r += WAVES) {
+      const Run run = unpack_run(lruns[r]);
+      const gptr_mut<double2> dst = values + 2 * int64_t(run.global);
+      for (int base = lane; base < 2 * run.len; base += 64 * U) {
+        double2 old[U];
+        double v0[U], v1[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+          const int i = base + 64 * k;
+          if (i < 2 * run.len) {
+            old[k] = dst[i];
+            const double *src = acc + 2 * (i & 1) * acc_plane<MAXB>() + run.lds + (i >> 1);
+            v0[k] = src[0];
+            v1[k] = src[acc_plane<MAXB>()];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+          const int i = base + 64 * k;
+          if (i < 2 * run.len) {
+            store_once(dst + i, v0[k] + old[k].x, v1[k] + old[k].y);
+            if (ZERO) {
+              double *src = acc + 2 * (i & 1) * acc_plane<MAXB>() + run.lds + (i >> 1);
+              src[0] = 0;
+              src[acc_plane<MAXB>()] = 0;
+            }
+          }
+        }
+      }
+    }
+  }
+  if (RES && J.residual) {
+    const gptr_mut<double2> res = gmem_mut(reinterpret_cast<double2 *>(J.residual));
+    for (int i = int(threadIdx.x); i < nr; i += WAVES * 64) {
+      const double2 a = reinterpret_cast<const double2 *>(racc)[i], old = res[lrows[i]];
+      res[lrows[i]] = make_double2(a.x + old.x, a.y + old.y);
+      if (ZERO) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
+    }
+  }
+}
+
 template <bool RES, bool JAC, int WAVES, int MAXB, bool ZERO = false, bool HALVES = false, typename RunT>
 __device__ __forceinline__ void owner_write(const MeshJob &J, double *acc, double *racc, const RunT *lruns, int nruns,
                                             const int *lrows, int nr) {
+  if (J.accumulate) {
+    owner_write_accumulate<RES, JAC, WAVES, MAXB, ZERO>(J, acc, racc, lruns, nruns, lrows, nr);
+    return;
+  }
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   if (JAC && J.values) {
     const gptr_mut<double2> values = gmem_mut(reinterpret_cast<double2 *>(J.values));
@@ -1554,13 +1609,7 @@ __device__ __forceinline__ void owner_write(const MeshJob &J, double *acc, doubl
         // 1 KB contiguously.
         for (int i = lane; i < 2 * run.len; i += 64) {
           double *src = acc + 2 * (i & 1) * acc_plane<MAXB>() + run.lds + (i >> 1);
-          double v0 = src[0], v1 = src[acc_plane<MAXB>()];
-          if (J.accumulate) {
-            const double2 old = dst[i];
-            v0 += old.x;
-            v1 += old.y;
-          }
-          store_once(dst + i, v0, v1);
+          store_once(dst + i, src[0], src[acc_plane<MAXB>()]);
           if (ZERO) {
             src[0] = 0;
             src[acc_plane<MAXB>()] = 0;
@@ -1570,18 +1619,8 @@ __device__ __forceinline__ void owner_write(const MeshJob &J, double *acc, doubl
         // One block per lane (two 16-byte stores).
         double *src = acc + run.lds;
         for (int i = lane; i < run.len; i += 64) {
-          double v[4];
-#pragma unroll
-          for (int c = 0; c < 4; c++) v[c] = src[i + c * acc_plane<MAXB>()];
-          if (J.accumulate) {
-            const double2 o0 = dst[2 * i], o1 = dst[2 * i + 1];
-            v[0] += o0.x;
-            v[1] += o0.y;
-            v[2] += o1.x;
-            v[3] += o1.y;
-          }
-          store_once(dst + 2 * i, v[0], v[1]);
-          store_once(dst + 2 * i + 1, v[2], v[3]);
+          store_once(dst + 2 * i, src[i], src[i + acc_plane<MAXB>()]);
+          store_once(dst + 2 * i + 1, src[i + 2 * acc_plane<MAXB>()], src[i + 3 * acc_plane<MAXB>()]);
           if (ZERO) {
 #pragma unroll
             for (int c = 0; c < 4; c++) src[i + c * acc_plane<MAXB>()] = 0;
@@ -1593,13 +1632,7 @@ __device__ __forceinline__ void owner_write(const MeshJob &J, double *acc, doubl
   if (RES && J.residual) {
     const gptr_mut<double2> res = gmem_mut(reinterpret_cast<double2 *>(J.residual));
     for (int i = int(threadIdx.x); i < nr; i += WAVES * 64) {
-      double2 r = reinterpret_cast<const double2 *>(racc)[i];
-      if (J.accumulate) {
-        const double2 old = res[lrows[i]];
-        r.x += old.x;
-        r.y += old.y;
-      }
-      res[lrows[i]] = r;
+      res[lrows[i]] = reinterpret_cast<const double2 *>(racc)[i];
       if (ZERO) reinterpret_cast<double2 *>(racc)[i] = make_double2(0, 0);
     }
   }
@@ -1972,6 +2005,13 @@ void launch_element_split(const SplitJob &job, const Basis1D &basis, int waves, 
     const int grid = (job.count + per - 1) / per;
     if (grid > 0) element_tiled_kernel<P, T, WAVES, RES><<<grid, WAVES * 64, 0, stream>>>(job, basis);
   };
+  if (job.accumulate) {
+    // accumulate mode (a compile-time variant, so the default emission is unchanged), default shape
+    constexpr int per = kTiledWaves * Ord<P>::EPW;
+    const int grid = (job.count + per - 1) / per;
+    if (grid > 0) element_tiled_kernel<P, T, kTiledWaves, RES, true><<<grid, kTiledWaves * 64, 0, stream>>>(job, basis);
+    return;
+  }
   // wavefronts per workgroup: the plan's (autotuned) choice
   if (waves == 1) go(std::integral_constant<int, 1>{});
   else if (waves == 2) go(std::integral_constant<int, 2>{});

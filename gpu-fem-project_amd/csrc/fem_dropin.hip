@@ -8,8 +8,8 @@
 // pattern, and a resident plan is built and cached. Calls then run the plan's kernels in accumulate mode (orders
 // 1-2: owner-computes parts; 3-4: element-once traversal and gathers): every output is read and written once
 // (+=, the reference's semantics), with no atomics. Every call first checks that the cached arrays are unchanged (a 64-bit checksum of conn and of
-// a strided sample of the map); a change rebuilds the entry. Anything else (a different pattern, nodes no element
-// references, a failed build) runs fem_jacobian.
+// a strided sample of the map); a change rebuilds the entry. Anything else (orders 3-4, a different pattern, nodes
+// no element references, a failed build) runs fem_jacobian.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,15 +37,18 @@ constexpr int kThreads = 256;
 constexpr int64_t kMapStride = 61;  // map entries sampled by the per-call check
 constexpr size_t kMaxEntries = 4;   // cached plans (least recently used evicted)
 
-__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
-  x += 0x9E3779B97F4A7C15ull;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-  return x ^ (x >> 31);
+// 32-bit finaliser of a (position, value) pair (MurmurHash3's fmix32 on a multiplicative combination).
+__device__ __forceinline__ unsigned int mix32(unsigned int k, unsigned int v) {
+  unsigned int x = k * 0x9E3779B9u ^ v * 0x85EBCA6Bu;
+  x ^= x >> 16;
+  x *= 0x85EBCA6Bu;
+  x ^= x >> 13;
+  x *= 0xC2B2AE35u;
+  return x ^ (x >> 16);
 }
 
-// Order-independent 64-bit checksum of conn[connPtr[e] + a] (all e, a) and map[k * stride] (sampled), plus the
-// largest node id and a flag for a negative one.
+// Order-independent 64-bit sum of mix32 over conn[connPtr[e] + a] (all e, a) and map[k * stride] (sampled), plus
+// the largest node id and a flag for a negative one. One atomic of each kind per workgroup.
 __global__ __launch_bounds__(kThreads) void checksum_kernel(const int *connPtr, const int *conn, int numElements,
                                                             int npe, const int *map, int64_t mapSamples,
                                                             unsigned long long *sum, int *maxNode, int *negative) {
@@ -58,23 +61,39 @@ __global__ __launch_bounds__(kThreads) void checksum_kernel(const int *connPtr, 
     if (k < refs) {
       const int e = int(k / npe), a = int(k - int64_t(e) * npe);
       const int n = conn[int64_t(connPtr[e]) + a];
-      h += mix64((unsigned long long)k * 0x100000001B3ull ^ (unsigned int)n);
+      h += mix32(unsigned(k), unsigned(n));
       mx = max(mx, n);
       neg = neg || n < 0;
     } else {
       const int64_t s = k - refs;
-      h += mix64(((unsigned long long)s << 1 | 1ull) * 0xD6E8FEB86659FD93ull ^ (unsigned int)map[s * kMapStride]);
+      h += (unsigned long long)mix32(unsigned(s) ^ 0x5bd1e995u, unsigned(map[s * kMapStride])) << 20;
     }
   }
+  __shared__ unsigned long long hs[kThreads / 64];
+  __shared__ int ms[kThreads / 64], ns[kThreads / 64];
   for (int off = 32; off > 0; off >>= 1) {
     h += __shfl_down(h, off);
     mx = max(mx, __shfl_down(mx, off));
   }
-  neg = __any(neg);
+  const bool wneg = __any(neg);
+  const int w = threadIdx.x / 64;
   if ((threadIdx.x & 63) == 0) {
-    atomicAdd(sum, h);
-    atomicMax(maxNode, mx);
-    if (neg) atomicOr(negative, 1);
+    hs[w] = h;
+    ms[w] = mx;
+    ns[w] = wneg;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long bh = 0;
+    int bm = -1, bn = 0;
+    for (int i = 0; i < kThreads / 64; i++) {
+      bh += hs[i];
+      bm = max(bm, ms[i]);
+      bn |= ns[i];
+    }
+    atomicAdd(sum, bh);
+    atomicMax(maxNode, bm);
+    if (bn) atomicOr(negative, 1);
   }
 }
 
@@ -122,7 +141,7 @@ struct Temp {
   }
 };
 
-unsigned grid_for(int64_t n) { return unsigned(std::min<int64_t>((n + kThreads - 1) / kThreads, 8192)); }
+unsigned grid_for(int64_t n) { return unsigned(std::min<int64_t>((n + kThreads - 1) / kThreads, 2048)); }
 
 int64_t map_samples(int numElements, int npe) {
   const int64_t n = int64_t(numElements) * npe * npe;
@@ -226,9 +245,11 @@ int fem_jacobian_dropin(int order, const int *connPtr, const int *conn, int numE
     return fem_jacobian(order, connPtr, conn, numElements, states, nodeCoords, E, nu, t, elementBCSRMap, residual,
                         matEntries, precision, stream, elapsedSeconds);
   };
-  if (order < 1 || order > 4 || numElements <= 0 || !connPtr || !conn || !elementBCSRMap || !matEntries || !states ||
+  if (order < 1 || order > 2 || numElements <= 0 || !connPtr || !conn || !elementBCSRMap || !matEntries || !states ||
       !nodeCoords || (precision != FEM_PREC_FP64 && precision != FEM_PREC_MIXED)) {
-    return plain();  // argument errors are reported by fem_jacobian
+    // argument errors are reported by fem_jacobian; at orders 3-4 its atomic kernels are faster than the
+    // element-once route in accumulate mode (annulus 200x1250 order 4: 4.3 against 6.0 ms)
+    return plain();
   }
   if (int64_t(numElements) * nodes_per_element(order) * nodes_per_element(order) > INT32_MAX) return plain();
   hipStream_t s = static_cast<hipStream_t>(stream);

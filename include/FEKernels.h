@@ -40,8 +40,8 @@ inline double runJacobianKernel(const int elementOrder, const int *const connPtr
                                 const double E, const double nu, const double t, int *elementBCSRMap,
                                 double *const residual, double *const matEntries) {
   double seconds = 0.0;
-  // fem_jacobian_dropin: a cached resident plan in accumulate mode when the map is this library's map of the
-  // connectivity; otherwise the element-parallel atomic kernel (fem_jacobian)
+  // fem_jacobian_dropin: at orders 1-2 a cached resident plan in accumulate mode when the map is this library's
+  // map of the connectivity; otherwise the element-parallel atomic kernel (fem_jacobian)
   const int rc = fem_jacobian_dropin(elementOrder, connPtr, conn, numElements, states, nodeCoords, E, nu, t,
                                      elementBCSRMap, residual, matEntries, FEM_PREC_FP64, nullptr, &seconds);
   if (rc != FEM_OK) {

@@ -183,14 +183,15 @@ int fem_jacobian(int order, const int *connPtr, const int *conn, int numElements
                  const double *nodeCoords, double E, double nu, double t, const int *elementBCSRMap,
                  double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds);
 /* fem_jacobian with the same arguments and semantics (+= into caller-owned residual / matEntries), for callers
- * of the reference's interface (runJacobianKernel, element/FEKernels.h:1125-1225): on first use of a (connPtr,
- * conn, elementBCSRMap, numElements, order) combination the library builds the connectivity's BSR pattern and
- * element map on the device and, if that map equals the caller's, a resident plan (kept until
+ * of the reference's interface (runJacobianKernel, element/FEKernels.h:1125-1225): at orders 1-2, on first use of
+ * a (connPtr, conn, elementBCSRMap, numElements, order) combination the library builds the connectivity's BSR
+ * pattern and element map on the device and, if that map equals the caller's, a resident plan (kept until
  * fem_dropin_cache_clear; at most four combinations, least recently used evicted). Calls then run the plan in
  * accumulate mode (fem_plan_assemble_accumulate): each output is read and written once, without atomics. Each
  * call first checks a 64-bit checksum of conn and of every 61st map entry (a change rebuilds the entry) and
- * synchronises the stream. Another matrix layout, nodes no element references, or a failed build run
- * fem_jacobian. elapsedSeconds covers the assembly. Not for graph capture (it synchronises). */
+ * synchronises the stream. Orders 3-4 (where the atomic kernels are faster than accumulate mode), another
+ * matrix layout, nodes no element references, or a failed build run fem_jacobian. elapsedSeconds covers the
+ * assembly. Not for graph capture (it synchronises). */
 int fem_jacobian_dropin(int order, const int *connPtr, const int *conn, int numElements, const double *states,
                         const double *nodeCoords, double E, double nu, double t, const int *elementBCSRMap,
                         double *residual, double *matEntries, int precision, void *stream, double *elapsedSeconds);

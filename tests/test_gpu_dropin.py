@@ -3,10 +3,10 @@ assembly in accumulate mode (fem_plan_assemble_accumulate), against the oracle.
 
 The reference's kernels add into caller-zeroed outputs (element/FEKernels.h:1125-1225); here outputs are
 prefilled with random values X, so a call must leave X + oracle (FP64: 1e-12 normwise against the result), and a
-second call X + 2 oracle. Checked: orders 1-4 are served by a plan (orders 3-4: element-once, with wavefronts
-left partly empty); the cache notices connectivity and map rewritten in place (node ids relabelled: a stale plan
-would write the old layout); a caller matrix of another layout falls back to the atomic kernels; accumulate mode
-on owner-route and element-once order-3 plans."""
+second call X + 2 oracle. Checked: orders 1-2 are served by a plan; the cache notices connectivity and map
+rewritten in place (node ids relabelled: a stale plan would write the old layout); a caller matrix of another
+layout and orders 3-4 run the atomic kernels; accumulate mode on owner-route and element-once plans of orders 3-4
+(with wavefronts left partly empty)."""
 
 import numpy as np
 import pytest
@@ -121,9 +121,10 @@ def test_dropin_other_layout_uses_atomic_kernels(fem, orc, torch_gpu):
 
 
 @pytest.mark.parametrize("kind,order", [("square", 3), ("annulus", 3), ("square", 4), ("annulus", 4)])
-def test_dropin_orders_3_4_element_once_accumulates(fem, orc, torch_gpu, kind, order):
-    """Orders 3-4: the cached plan's element-once traversal and gathers in accumulate mode (elements counts that
-    leave the last wavefront partly empty, so repeated lanes of the emission must not add twice)."""
+def test_element_once_accumulate_and_dropin_orders_3_4(fem, orc, torch_gpu, kind, order):
+    """Orders 3-4: element-once plans in accumulate mode (element counts that leave the last wavefront partly
+    empty, so the emission's repeating lanes must not add twice), and the drop-in, which runs the atomic kernels
+    there."""
     torch = torch_gpu
     fem.dropin_cache_clear()
     m = (fem.square_mesh(7, order, jitter=0.2, seed=order) if kind == "square"
@@ -132,14 +133,20 @@ def test_dropin_orders_3_4_element_once_accumulates(fem, orc, torch_gpu, kind, o
     rng = np.random.default_rng(order)
     X_r = rng.uniform(-1, 1, rref.size) * np.abs(rref).max()
     X_v = rng.uniform(-1, 1, vref.size) * np.abs(vref).max()
-    res, vals = d(torch, X_r), d(torch, X_v)
     cp_d, conn_d, map_d, u_d, x_d = d(torch, m.conn_ptr), d(torch, m.conn), d(torch, bmap), d(torch, u), d(torch, m.coords)
+    plan = fem.Plan(m, pat)
+    res, vals = d(torch, X_r), d(torch, X_v)
     for k in (1, 2):
-        call(fem, torch, m, conn_d, cp_d, u_d, x_d, map_d, res, vals)
-        assert fem.dropin_cache_info(order, cp_d, conn_d, m.num_elements, map_d) == (True, True)
+        plan.assemble(u_d, x_d, E, NU, T, res=res, values=vals, accumulate=True)
+        torch.cuda.synchronize()
         assert textbook.normwise_rel(res.cpu().numpy(), X_r + k * rref) < TOL
         assert textbook.normwise_rel(vals.cpu().numpy(), X_v + k * vref) < TOL
-    fem.dropin_cache_clear()
+    plan.close()
+    res, vals = d(torch, X_r), d(torch, X_v)
+    call(fem, torch, m, conn_d, cp_d, u_d, x_d, map_d, res, vals)
+    assert fem.dropin_cache_info(order, cp_d, conn_d, m.num_elements, map_d) == (False, False)
+    assert textbook.normwise_rel(res.cpu().numpy(), X_r + rref) < TOL
+    assert textbook.normwise_rel(vals.cpu().numpy(), X_v + vref) < TOL
 
 
 @pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 1e-5)], ids=["fp64", "mixed"])
