@@ -2741,6 +2741,7 @@ struct fem_batch {
   hipEvent_t staged = nullptr;  // recorded after the copy: the host mirror is rewritten only once it has run
   bool stagedPending = false;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // per precision
+  bool assembled = false;                          // fem_batch_assemble has staged job tables
   hipStream_t captureStream = nullptr;             // graphs are recorded here and replayed on any stream
 
   int *dCounts() const { return reinterpret_cast<int *>(dTables); }
@@ -2851,7 +2852,15 @@ int fem::batch_prepare_graph(fem_batch *b, int precision) {
     return set_error(FEM_ERR_INVALID_ARGUMENT, "unknown precision mode %d", precision);
   }
   if (!(b->flags & FEM_BATCH_GRAPH) || b->graph[precision]) return FEM_OK;
-  return capture_batch_graph(b, precision);
+  int rc = capture_batch_graph(b, precision);
+  if (rc) return rc;
+  if (!b->assembled) {
+    // One replay while every job count in the tables is still zero (all kernels exit at once): the runtime's
+    // first-launch allocations (scratch of kernels that spill, graph kernel arguments) happen now, not later.
+    FEM_HIP_CHECK(hipGraphLaunch(b->graph[precision], b->captureStream));
+    FEM_HIP_CHECK(hipStreamSynchronize(b->captureStream));
+  }
+  return FEM_OK;
 }
 
 extern "C" {
@@ -3062,6 +3071,7 @@ int fem_batch_assemble(fem_batch *batch, int precision, void *stream, double *el
   FEM_HIP_CHECK(hipMemcpyAsync(batch->dTables, batch->hTables, copyEnd, hipMemcpyHostToDevice, s));
   FEM_HIP_CHECK(hipEventRecord(batch->staged, s));
   batch->stagedPending = true;
+  batch->assembled = true;
   if (useGraph && !batch->graph[precision]) {
     int rc = capture_batch_graph(batch, precision);
     if (rc) return rc;

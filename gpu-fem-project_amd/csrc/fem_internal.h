@@ -102,7 +102,9 @@ int batch_add_staged(fem_batch *batch, fem_plan *plan, const double *states, con
                      double nu, double t, double *residual, double *matEntries, double *stage);
 bool plan_streams_public(const fem_plan *plan);
 // Records a graph batch's launch graph for `precision` now (it does not depend on the batch's contents), so that
-// no capture is in flight later, when other threads may be using the device. No-op without FEM_BATCH_GRAPH.
+// no capture is in flight later, when other threads may be using the device, and (before the batch's first
+// assembly) replays it once with no jobs, so the runtime's first-launch allocations happen now too. No-op without
+// FEM_BATCH_GRAPH.
 int batch_prepare_graph(fem_batch *batch, int precision);
 // fem_plan_assemble (accumulate false) and fem_plan_assemble_accumulate (true).
 int plan_assemble(fem_plan *plan, const double *states, const double *nodeCoords, double E, double nu, double t,
