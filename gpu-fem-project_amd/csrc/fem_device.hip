@@ -810,8 +810,11 @@ __device__ __forceinline__ void tiled_wave(WaveLDSTile<P, T> &W, const Basis1D &
 }
 
 
+// Two wavefronts per SIMD, which the LDS of two 4-wavefront workgroups per CU allows: without the bound the FP64
+// order-3 kernel takes 259 registers and runs one wavefront per SIMD (order 3 FP64: 1.40 against 1.50 ms).
 template <int P, typename T, int WAVES, bool RES, bool ACC = false>
-__global__ __launch_bounds__(WAVES * 64) void element_tiled_kernel(const SplitJob job, const Basis1D basisIn) {
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(2)))
+void element_tiled_kernel(const SplitJob job, const Basis1D basisIn) {
   using O = Ord<P>;
   __shared__ WaveLDSTile<P, T> wl[WAVES];
   __shared__ Basis1D basis;
