@@ -8,8 +8,8 @@
 // pattern, and a resident plan is built and cached. Calls then run the plan's kernels in accumulate mode (orders
 // 1-2: owner-computes parts; 3-4: element-once traversal and gathers): every output is read and written once
 // (+=, the reference's semantics), with no atomics. Every call first checks that the cached arrays are unchanged (a 64-bit checksum of conn and of
-// a strided sample of the map); a change rebuilds the entry. Anything else (orders 3-4, a different pattern, nodes
-// no element references, a failed build) runs fem_jacobian.
+// a strided sample of the map); a change rebuilds the entry. Anything else (orders 3-4, a different pattern, a failed
+// build) runs fem_jacobian.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -224,7 +224,7 @@ int build(Entry &e, const double *nodeCoords, hipStream_t stream) {
   rc = fem_plan_create(e.order, numNodes, e.numElements, hConnPtr.data(), hConn.data(), hRowp.data(), hCols.data(),
                        hCoords.data(), 0, &plan);
   if (rc) {
-    clear_error();  // e.g. nodes no element references: fem_jacobian serves the mesh
+    clear_error();  // e.g. a row beyond the plan's 256-block limit: fem_jacobian serves the mesh
     return FEM_OK;
   }
   e.plan = plan;

@@ -270,23 +270,31 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
   const int rowBudget = residualOnly ? maxRows : std::max(1, int(maxRows * scale));
   std::vector<int> nptr, nadj;
   node_element_adjacency(numNodes, numElements, N, connPtr, conn, nptr, nadj);
-  for (int n = 0; n < numNodes; n++) {
-    if (nptr[n + 1] == nptr[n]) {
-      return set_error(FEM_ERR_INVALID_ARGUMENT, "node %d belongs to no element; renumber the mesh without it", n);
-    }
+  // Nodes that belong to no element keep zero blocks and zero residual entries. The cut runs over the other
+  // nodes, so every part visits at least one element; each of them then joins the first part (in cut order)
+  // with room for its row, whose write-out stores the zeros.
+  std::vector<int> isolated;
+  {
+    std::vector<int> kept;
+    kept.reserve(size_t(numNodes));
+    for (int n : order_nodes) (nptr[n + 1] == nptr[n] ? isolated : kept).push_back(n);
+    order_nodes.swap(kept);
   }
-  out.partRowPtr.push_back(0);
+  const int numCut = int(order_nodes.size());
+  std::vector<int> cutPtr{0}, partRowCount, partBlockCount;
   {
     std::vector<int> stamp(numElements, -1);
     int blocks = 0, rowsInPart = 0, visits = 0, part = 0;
-    for (int k = 0; k < numNodes; k++) {
+    for (int k = 0; k < numCut; k++) {
       const int n = order_nodes[k];
       const int len = rowp[n + 1] - rowp[n];
       int newVisits = 0;
       for (int j = nptr[n]; j < nptr[n + 1]; j++) newVisits += stamp[nadj[size_t(j)]] != part;
       if (rowsInPart > 0 &&
           (blocks + len > blockBudget || rowsInPart + 1 > rowBudget || visits + newVisits > maxVisits)) {
-        out.partRowPtr.push_back(k);
+        cutPtr.push_back(k);
+        partRowCount.push_back(rowsInPart);
+        partBlockCount.push_back(blocks);
         part++;
         blocks = 0;
         rowsInPart = 0;
@@ -298,7 +306,34 @@ int build_plan_host(int order, int numNodes, int numElements, const int *connPtr
       rowsInPart++;
       visits += newVisits;
     }
-    out.partRowPtr.push_back(numNodes);
+    cutPtr.push_back(numCut);
+    partRowCount.push_back(rowsInPart);
+    partBlockCount.push_back(blocks);
+  }
+  {
+    const int cutParts = int(cutPtr.size()) - 1;
+    std::vector<std::vector<int>> joined(static_cast<size_t>(cutParts));
+    int p = 0;
+    for (int n : isolated) {
+      const int len = rowp[n + 1] - rowp[n];
+      while (p < cutParts && (partRowCount[p] + 1 > rowBudget || partBlockCount[p] + len > blockBudget)) p++;
+      if (p == cutParts) {
+        return set_error(FEM_ERR_UNSUPPORTED, "no part has room for the row of node %d, which belongs to no element",
+                         n);
+      }
+      joined[size_t(p)].push_back(n);
+      partRowCount[p]++;
+      partBlockCount[p] += len;
+    }
+    std::vector<int> placed;
+    placed.reserve(size_t(numNodes));
+    out.partRowPtr.assign(1, 0);
+    for (int q = 0; q < cutParts; q++) {
+      placed.insert(placed.end(), order_nodes.begin() + cutPtr[q], order_nodes.begin() + cutPtr[q + 1]);
+      placed.insert(placed.end(), joined[size_t(q)].begin(), joined[size_t(q)].end());
+      out.partRowPtr.push_back(int(placed.size()));
+    }
+    order_nodes.swap(placed);
   }
   out.numParts = int(out.partRowPtr.size()) - 1;
   const int P = out.numParts;

@@ -190,7 +190,7 @@ int fem_jacobian(int order, const int *connPtr, const int *conn, int numElements
  * accumulate mode (fem_plan_assemble_accumulate): each output is read and written once, without atomics. Each
  * call first checks a 64-bit checksum of conn and of every 61st map entry (a change rebuilds the entry) and
  * synchronises the stream. Orders 3-4 (where the atomic kernels are faster than accumulate mode), another
- * matrix layout, nodes no element references, or a failed build run fem_jacobian. elapsedSeconds covers the
+ * matrix layout, or a failed build run fem_jacobian. elapsedSeconds covers the
  * assembly. Not for graph capture (it synchronises). */
 int fem_jacobian_dropin(int order, const int *connPtr, const int *conn, int numElements, const double *states,
                         const double *nodeCoords, double E, double nu, double t, const int *elementBCSRMap,

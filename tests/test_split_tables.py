@@ -2,7 +2,8 @@
 checked on the host: assembling random element contributions through the tables (exclusive outputs stored,
 the others staged and summed in slot order) reproduces a plain scatter-add in element order bit for bit, and
 writes every BSR block and residual entry exactly once. Meshes: every order, square and annulus, shuffled
-node and element numbering, a high-valence fan."""
+node and element numbering, a high-valence fan, nodes that no element references (their outputs are
+gathered from no contributions: zero)."""
 
 import numpy as np
 import pytest
@@ -22,6 +23,16 @@ def fan(fem, k):
     return fem.Mesh(1, np.random.default_rng(k).random((1 + 3 * k, 2)), conn, f"fan of {k}")
 
 
+def with_isolated(fem, mesh, extra, seed):
+    """`extra` nodes that no element references, scattered through the numbering."""
+    rng = np.random.default_rng(seed)
+    n0 = mesh.num_nodes
+    perm = rng.permutation(n0 + extra).astype(np.int32)
+    coords = np.zeros((n0 + extra, 2))
+    coords[perm[:n0]] = mesh.coords
+    return fem.Mesh(mesh.order, coords, np.ascontiguousarray(perm[mesh.conn]), mesh.kind + " + isolated nodes")
+
+
 def meshes(fem):
     out = []
     for p in (1, 2, 3, 4):
@@ -30,6 +41,8 @@ def meshes(fem):
         out.append(permuted(fem, fem.square_mesh(4, p), 10 + p))
     out.append(fan(fem, 40))
     out.append(fem.square_mesh(1, 4))
+    out.append(with_isolated(fem, fem.square_mesh(3, 2), 6, 21))
+    out.append(with_isolated(fem, fem.square_mesh(3, 4), 6, 22))
     return out
 
 
@@ -49,7 +62,7 @@ def split_assemble(t, contrib, outputs, width):
     return out, excl
 
 
-@pytest.mark.parametrize("idx", range(14))
+@pytest.mark.parametrize("idx", range(16))
 def test_split_tables_reproduce_scatter_add(fem, idx):
     mesh = meshes(fem)[idx]
     pat = fem.bsr_pattern(mesh)
