@@ -2744,6 +2744,15 @@ struct fem_batch {
   hipEvent_t staged = nullptr;  // recorded after the copy: the host mirror is rewritten only once it has run
   bool stagedPending = false;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // per precision
+  // Jobs of deterministic and residual-only plans (batches without graphs only): each runs as its plan's own
+  // assembly on the batch's stream, after the groups.
+  struct Solo {
+    fem_plan *plan;
+    const double *states, *coords;
+    double E, nu, t;
+    double *residual, *values;
+  };
+  std::vector<Solo> solo;
   bool assembled = false;                          // fem_batch_assemble has staged job tables
   hipStream_t captureStream = nullptr;             // graphs are recorded here and replayed on any stream
 
@@ -2915,6 +2924,7 @@ int fem_batch_clear(fem_batch *batch) {
     g.stageOff.clear();
     g.rstageOff.clear();
   }
+  batch->solo.clear();
   batch->splitDoubles = 0;
   batch->numJobs = 0;
   return FEM_OK;
@@ -2941,12 +2951,23 @@ int batch_add_staged(fem_batch *batch, fem_plan *plan, const double *states, con
   if (!batch || !plan) return set_error(FEM_ERR_NULL_POINTER, "batch/plan is NULL");
   if (!states || !nodeCoords) return set_error(FEM_ERR_NULL_POINTER, "states/nodeCoords is NULL");
   if (!residual && !matEntries) return set_error(FEM_ERR_NULL_POINTER, "residual and matEntries are both NULL");
-  if ((plan->flags & FEM_PLAN_DETERMINISTIC) || (plan->flags & FEM_PLAN_RESIDUAL_ONLY)) {
-    return set_error(FEM_ERR_UNSUPPORTED, "batches take plans built with default flags");
-  }
   Material m;
   int rc = make_material(E, nu, t, m);
   if (rc) return rc;
+  if ((plan->flags & FEM_PLAN_DETERMINISTIC) || (plan->flags & FEM_PLAN_RESIDUAL_ONLY)) {
+    if (batch->flags & FEM_BATCH_GRAPH) {
+      return set_error(FEM_ERR_UNSUPPORTED, "graph batches take plans built with default flags");
+    }
+    if ((plan->flags & FEM_PLAN_RESIDUAL_ONLY) && matEntries) {
+      return set_error(FEM_ERR_INVALID_ARGUMENT, "plan was built residual-only; it cannot assemble the Jacobian");
+    }
+    if (int(batch->solo.size()) >= batch->maxJobs) {
+      return set_error(FEM_ERR_INVALID_ARGUMENT, "batch group is full (%d jobs)", batch->maxJobs);
+    }
+    batch->solo.push_back(fem_batch::Solo{plan, states, nodeCoords, E, nu, t, residual, matEntries});
+    batch->numJobs++;
+    return FEM_OK;
+  }
   // Jacobian jobs of plans with element-once tables take the element-once groups; their staging is placed
   // in the batch's arena at assembly.
   if (plan->dConn && matEntries) {
@@ -3104,6 +3125,11 @@ int fem_batch_assemble(fem_batch *batch, int precision, void *stream, double *el
     }
     for (int g = 0; g < fem_batch::kSplitGroups; g++) {
       if (!batch->split[g].jobs.empty()) launch_split_group(batch, g, precision, false, s);
+    }
+    for (const fem_batch::Solo &j : batch->solo) {
+      const int rc = fem::plan_assemble(j.plan, j.states, j.coords, j.E, j.nu, j.t, j.residual, j.values, precision, s,
+                                        nullptr, false);
+      if (rc) return rc;
     }
     return FEM_OK;
   });

@@ -276,9 +276,12 @@ int fem_plan_destroy(fem_plan *plan);
  * and residual/no-residual group, then the gathers, with staging in the batch's own device arena (grown
  * at assembly when the batch's jobs need more, after a device synchronisation), so batches holding the
  * same plans may run concurrently on different streams.
- * Pointers are the same as fem_plan_assemble's; outputs are overwritten. Plans must be built with
- * flags 0. fem_batch_assemble uploads the job tables (asynchronously on `stream`) and launches; the
- * batch may be cleared and refilled once the call returns. */
+ * Pointers are the same as fem_plan_assemble's; outputs are overwritten. Jobs of plans built with
+ * FEM_PLAN_DETERMINISTIC or FEM_PLAN_RESIDUAL_ONLY run as their plan's own assembly on `stream` after the
+ * groups (batches without FEM_BATCH_GRAPH only: graph batches reject them); they use their plan's staging,
+ * so batches holding the same such plan must not run concurrently. fem_batch_assemble uploads the job
+ * tables (asynchronously on `stream`) and launches; the batch may be cleared and refilled once the call
+ * returns. */
 typedef struct fem_batch fem_batch;
 /* maxJobs: capacity per (order, outputs) group. flags: FEM_BATCH_GRAPH replays the twelve group
  * launches as one captured hipGraph whose kernels read their job counts from device memory. */

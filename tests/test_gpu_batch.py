@@ -84,15 +84,50 @@ def test_batch_mixed_orders_and_outputs(fem, torch_gpu, meshes):
         plan.close()
 
 
-def test_batch_rejects_deterministic_plans(fem, torch_gpu, meshes):
-    c = meshes[1]
-    plan = fem.Plan(c["mesh"], c["pat"], flags=fem.FEM_PLAN_DETERMINISTIC)
+def test_batch_deterministic_and_residual_only_plans(fem, torch_gpu, meshes):
+    """Batches without graphs run jobs of deterministic and residual-only plans as the plan's own assembly
+    after the groups: their outputs equal the plan assembled alone (deterministic plans bit for bit) and the
+    oracle; graph batches reject such jobs."""
+    torch = torch_gpu
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")  # noqa: E731
+    nan = lambda n: torch.full((n,), float("nan"), dtype=torch.float64, device="cuda:0")  # noqa: E731
     batch = fem.Batch()
-    t = torch_gpu.zeros(8, dtype=torch_gpu.float64, device="cuda:0")
+    jobs = []
+    for i, c in enumerate(meshes[:6]):
+        m = c["mesh"]
+        flags = [fem.FEM_PLAN_DETERMINISTIC, fem.FEM_PLAN_RESIDUAL_ONLY, 0][i % 3]
+        print(f"batch with special plans: mesh {i}, order {m.order}, flags {flags}", flush=True)
+        plan = fem.Plan(m, c["pat"], flags=flags)
+        coords, states = d(m.coords), d(c["u"])
+        res = nan(2 * m.num_nodes)
+        vals = None if flags == fem.FEM_PLAN_RESIDUAL_ONLY else nan(4 * c["pat"].num_blocks)
+        if flags == fem.FEM_PLAN_RESIDUAL_ONLY:
+            with pytest.raises(fem.FemError):
+                batch.add(plan, states, coords, E, NU, T, res=res, values=nan(4 * c["pat"].num_blocks))
+        batch.add(plan, states, coords, E, NU, T, res=res, values=vals)
+        jobs.append((c, plan, flags, coords, states, res, vals))
+    batch.assemble()
+    torch.cuda.synchronize()
+    for c, plan, flags, coords, states, res, vals in jobs:
+        r = res.cpu().numpy()
+        assert textbook.normwise_rel(r, c["res"]) < 1e-12
+        r1 = nan(r.size)
+        v1 = None if vals is None else nan(vals.numel())
+        plan.assemble(states, coords, E, NU, T, res=r1, values=v1)
+        torch.cuda.synchronize()
+        if vals is not None:
+            v = vals.cpu().numpy()
+            assert textbook.normwise_rel(v, c["vals"]) < 1e-12
+        if flags == fem.FEM_PLAN_DETERMINISTIC:
+            assert np.array_equal(r, r1.cpu().numpy()) and np.array_equal(v, v1.cpu().numpy())
+    graph = fem.Batch(graph=True)
+    c, plan, _, coords, states, res, vals = jobs[0]
     with pytest.raises(fem.FemError):
-        batch.add(plan, t, t, E, NU, T, res=t)
+        graph.add(plan, states, coords, E, NU, T, res=res, values=vals)
+    graph.close()
     batch.close()
-    plan.close()
+    for job in jobs:
+        job[1].close()
 
 
 @pytest.mark.parametrize("graph", [False, True])
